@@ -1,0 +1,172 @@
+/*
+ * radm2.h -- C ABI of the MI355X-native RADM2 Rosenbrock integrator.
+ *
+ * One call integrates many independent grid cells of the RADM2 mechanism
+ * (59 variable species, 2 fixed, 156 reactions) over [0, dt] with the KPP
+ * Ros3 method, one cell per GPU lane, entirely on the device.
+ *
+ * The reference (Rizkyanno/RADM2-Solver-Multi-GPU, a single CUDA file) has no
+ * library boundary: its integrator is the KPP C driver
+ *     int Rosenbrock(double Tstart, double Tend, double *AbsTol,
+ *                    double *RelTol, double RPAR[20], int IPAR[20]);
+ * (mgpu.cu:3664-3848), reached from INTEGRATE() (mgpu.cu:3850-3889), with the
+ * cell state passed through globals (DataDevice[0].d_VAR / d_FIX / d_RCONST,
+ * mgpu.cu:263-292) filled by dev_convert / dev_Update_RCONST
+ * (mgpu.cu:513-646, 752-919).  Each entry point below names the reference
+ * interface it replaces.
+ *
+ * Conventions
+ *  - Arrays are device pointers, structure-of-arrays with the cell index
+ *    fastest: element (s, p) of a [S][ld] array is x[s*ld + p], p < ncells.
+ *  - Species order is the reference's ind_* order (mgpu.cu:125-183); the 21
+ *    photolysis rates are in the reference's jv order (mgpu.cu:763-783), s^-1.
+ *  - Calls are asynchronous on the context's stream; radm2_sync() waits.
+ *  - Call-level return: 0 on success; a KPP option error -1..-5 (raised
+ *    before any launch, as mgpu.cu:3715-3811 does); or a RADM2_E_* code.
+ *    Per-cell failures (-6 too many steps, -7 step too small, -8 singular
+ *    matrix) go to status[p] and never affect other cells.
+ *  - Contexts are not thread-safe; use one per host thread.
+ */
+#ifndef RADM2_H
+#define RADM2_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RADM2_NVAR 59   /* variable species (mgpu.cu:18) */
+#define RADM2_NFIX 2    /* fixed species H2O, M (mgpu.cu:20, 185-189) */
+#define RADM2_NREACT 156 /* reactions (mgpu.cu:21) */
+#define RADM2_NJ 21     /* photolysis inputs used by the rate constants */
+#define RADM2_LU_NNZ 659 /* LU nonzeros (mgpu.cu:25) */
+
+/* call-level error codes (KPP's -1..-5 are returned as is) */
+#define RADM2_OK 0
+#define RADM2_E_ARG (-101)         /* null pointer, bad size or stride */
+#define RADM2_E_HIP (-102)         /* HIP runtime error */
+#define RADM2_E_UNSUPPORTED (-103) /* option outside what the solver implements */
+
+/* per-cell status codes (KPP IERR, mgpu.cu:976-1005) */
+#define RADM2_CELL_OK 1
+#define RADM2_CELL_TOO_MANY_STEPS (-6)
+#define RADM2_CELL_STEP_TOO_SMALL (-7)
+#define RADM2_CELL_SINGULAR (-8)
+
+/* schedules */
+#define RADM2_SCHED_GRID 0       /* one-shot grid: one lane per cell */
+#define RADM2_SCHED_PERSISTENT 1 /* resident waves, work stealing over cells */
+
+/*
+ * Integration options, mirroring RPAR[0..6] and IPAR[2..3] of the
+ * reference's Rosenbrock() (mgpu.cu:3698-3791): a zero field selects KPP's
+ * default (Hmin 0, Hmax dt, Hstart max(Hmin,1e-6), FacMin 0.2, FacMax 6,
+ * FacRej 0.1, FacSafe 0.9, 100000 steps); negative values are errors.
+ * method: 0 or 2 = Ros3, the only method the reference implements.
+ * Passing NULL instead selects INTEGRATE()'s call (mgpu.cu:3866-3873):
+ * all defaults except Hstart = 0.01*dt.
+ */
+typedef struct radm2_opts {
+  double hmin, hmax, hstart, facmin, facmax, facrej, facsafe;
+  int32_t max_steps, method;
+} radm2_opts;
+
+typedef struct radm2_ctx radm2_ctx;
+typedef struct radm2_plan radm2_plan;
+
+/* One batch member for radm2_integrate_batched / radm2_plan_create. */
+typedef struct radm2_domain {
+  int64_t ncells, ld;
+  double *conc;           /* [59][ld] concentrations (molec/cm3), in -> out */
+  const double *temp;     /* [ld] temperature (K) */
+  const double *cair;     /* [ld] air number density M (molec/cm3) */
+  const double *h2o;      /* [ld] water (molec/cm3) */
+  const double *jrate;    /* [21][ld] photolysis rates (s^-1) */
+  const double *rc_n2o5;  /* [ld] N2O5 hydrolysis rate, NULL = reference value at RH 0 */
+  const double *h0;       /* [ld] initial step per cell, NULL = options' Hstart */
+  int32_t *nstep, *nacc, *nrej, *status; /* [ld] per-cell outputs, each nullable */
+  double *hexit;          /* [ld] step size at exit (nullable) */
+  double *hfirst;         /* [ld] controller proposal after the first accepted step (nullable) */
+  double dt, rtol, atol;
+  const radm2_opts *opts; /* NULL = INTEGRATE() defaults */
+} radm2_domain;
+
+/* Create a context on HIP device `device`.  stream: a hipStream_t to run on,
+ * or NULL for a context-owned stream.  Replaces the reference's device setup
+ * (initializeWRFChemData, mgpu.cu:419-511) for the solver's own needs. */
+int radm2_create(radm2_ctx **ctx, int device, void *stream);
+void radm2_destroy(radm2_ctx *ctx);
+
+/* Schedule for later calls: RADM2_SCHED_GRID or RADM2_SCHED_PERSISTENT,
+ * block size (64, 128 or 256; 0 = tuned default), persistent refill
+ * threshold in idle lanes per wave (1..64; 0 = tuned default). */
+int radm2_set_schedule(radm2_ctx *ctx, int schedule, int block, int refill_min);
+
+/*
+ * Integrate ncells cells over [0, dt] with rtol/atol for every species.
+ * Replaces Rosenbrock() + RosenbrockIntegrator() (mgpu.cu:3664-3848,
+ * 3412-3662) as called by INTEGRATE() (mgpu.cu:3850-3889), with the rate
+ * constants of dev_Update_RCONST (mgpu.cu:752-919) evaluated inside the
+ * kernel from temp/cair/jrate/rc_n2o5, and the cell state passed explicitly
+ * instead of through globals.  Outputs: conc (end state), per-cell step,
+ * accepted and rejected counts, status and exit step size.
+ */
+int radm2_integrate(radm2_ctx *ctx, int64_t ncells, int64_t ld, double *conc,
+                    const double *temp, const double *cair, const double *h2o,
+                    const double *jrate, const double *rc_n2o5, double dt, double rtol,
+                    double atol, const radm2_opts *opts, const double *h0, int32_t *nstep,
+                    int32_t *nacc, int32_t *nrej, int32_t *status, double *hexit);
+
+/* Several independent domains (own arrays, dt, tolerances) in one launch.
+ * No reference counterpart beyond its single hard-coded 39x19x39 domain
+ * (mgpu.cu:420, 651-653); each domain follows radm2_integrate's semantics. */
+int radm2_integrate_batched(radm2_ctx *ctx, int ndomains, const radm2_domain *doms);
+
+/* Pre-validated batch captured as a HIP graph (counter reset + persistent
+ * launch) for repeated calls on the same buffers. */
+int radm2_plan_create(radm2_ctx *ctx, int ndomains, const radm2_domain *doms,
+                      radm2_plan **plan);
+int radm2_plan_launch(radm2_plan *plan);
+void radm2_plan_destroy(radm2_plan *plan);
+
+/* Wait for all work on the context's stream. */
+int radm2_sync(radm2_ctx *ctx);
+
+/* Device time in ms of the last integrate/launch call (HIP events on the
+ * context's stream; valid after radm2_sync). Replaces the reference's
+ * cudaStartTimer/cudaStopTimer (mgpu.cu:298-317). */
+int radm2_last_elapsed_ms(radm2_ctx *ctx, float *ms);
+
+/* Optional global counters summed over all cells of later calls: attempts,
+ * accepted, rejected, persistent refill rounds.  The reference keeps global
+ * Nstp/Nacc/Nrej (mgpu.cu:210, 3835-3842). */
+int radm2_enable_stats(radm2_ctx *ctx, int on);
+int radm2_read_stats(radm2_ctx *ctx, uint64_t out[4]);
+
+/* KPP option check without running: returns 0 or the IERR code -1..-5 of
+ * Rosenbrock() (mgpu.cu:3715-3811). */
+int radm2_check_opts(double dt, double rtol, double atol, const radm2_opts *opts);
+
+/*
+ * KPP front-end with the reference's own calling convention: Rosenbrock(
+ * Tstart, Tend, AbsTol, RelTol, RPAR, IPAR) (mgpu.cu:3664-3848) over the
+ * cells of `state` (its dt/rtol/atol/opts fields are ignored).  IPAR[0]
+ * must be 1 (autonomous; the reference's only use), IPAR[1] selects scalar
+ * (1) or vector (0) tolerances (vectors must hold one value for all 59
+ * species), IPAR[2..3] and RPAR[0..6] as radm2_opts.  On return IPAR[10..17]
+ * = Nfun, Njac, Nstp, Nacc, Nrej, Ndec, Nsol, Nsng summed over cells, and
+ * RPAR[10..11] = Texit, Hexit (Tend and the largest exit step when every
+ * cell succeeded).  Returns 1 (KPP success), -1..-8 or a RADM2_E_* code.
+ * Synchronous.
+ */
+int radm2_rosenbrock(radm2_ctx *ctx, const radm2_domain *state, double Tstart, double Tend,
+                     const double *AbsTol, const double *RelTol, double RPAR[20],
+                     int32_t IPAR[20]);
+
+/* Text for a call-level or per-cell code (ros_ErrorMsg, mgpu.cu:976-1005). */
+const char *radm2_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RADM2_H */

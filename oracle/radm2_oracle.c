@@ -1,0 +1,342 @@
+/*
+ * CPU oracle: restatement of the reference's per-cell RADM2 Rosenbrock
+ * (KPP Ros3) integrator in plain C.
+ *
+ * TEST INFRASTRUCTURE ONLY.  It is the parity checker for the HIP path (its
+ * functions are called only from tests/, the smoke check and the CPU-baseline
+ * leg of bench.py) and is never linked into or called by the product.
+ *
+ * Reference semantics followed (file:line of the reference's mgpu.cu):
+ *   rate-law helpers ARR2/TROE/TROEE/THERMAL_T2/k46  mgpu.cu:708-750
+ *   rate constants                                   mgpu.cu:763-918
+ *   tendency Fun (H2O2+OH loss omitted, :1275)        mgpu.cu:1103-1385
+ *   Jacobian (keeps the H2O2+OH loss)                 mgpu.cu:1685-2651
+ *   Ghimj = -Jac0 + 1/(H*gamma1) on the diagonal      mgpu.cu:3128-3149, 2685-2703
+ *   row-wise no-pivot LU, zero-diagonal check         mgpu.cu:2726-2780
+ *   forward/backward substitution                     mgpu.cu:2854-3096
+ *   Ros3 coefficients                                 mgpu.cu:1008-1062
+ *   stage formulas (the commented-out stage loop)     mgpu.cu:3595-3625
+ *   combination, error norm, accept/reject            mgpu.cu:3223-3260
+ *   time loop mask and H clamp                        mgpu.cu:1608-1613
+ *   initial step, H <= 10*Roundoff -> 1e-6            mgpu.cu:3475-3477
+ *   option defaults and validation                    mgpu.cu:3688-3811
+ *   default call (Hstart = 0.01*dt, Ros3, autonomous) mgpu.cu:3850-3873
+ *   unit roundoff (2^-52)                             mgpu.cu:949-973
+ * Resolutions of the reference's defects: per-cell state everywhere; the
+ * diagonal shift goes on the diagonal; FP64 M/E coefficients; the error
+ * scale is atol + rtol*max(|Y|,|Ynew|); KPP's -6 (step count) and -7 (step
+ * too small) checks are applied at the top of each time step.
+ *
+ * Arithmetic order contract shared with the HIP code generator: every
+ * accumulation is fma(c, term, acc) from 0 in table order; monomials are
+ * k times factors left to right; the LU stores inverse pivots unless the
+ * reference's division form is asked for.
+ */
+#include "radm2_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- Ros3 (mgpu.cu:1008-1062) -------------------------------------------- */
+static const double ROS_C21 = -1.0156171083877702091975600115545;
+static const double ROS_C31 = 4.0759956452537699824805835358067;
+static const double ROS_C32 = 9.2076794298330791242156818474003;
+static const double ROS_M1 = 1.0;
+static const double ROS_M2 = 6.1697947043828245592553615689730;
+static const double ROS_M3 = -0.4277225654321857332623837380651;
+static const double ROS_E1 = 0.5;
+static const double ROS_E2 = -2.9079558716805469821718236208017;
+static const double ROS_E3 = 0.2235406989781156962736090927619;
+static const double ROS_GAMMA1 = 0.43586652150845899941601945119356;
+static const double ROS_ELO = 3.0;
+static const double DELTA_MIN = 1.0e-6;
+
+#define O_MAX(a, b) (((a) >= (b)) ? (a) : (b))
+#define O_MIN(b, c) (((b) < (c)) ? (b) : (c))
+
+/* ---- rate-law helpers, as the reference writes them (mgpu.cu:708-750) ---- */
+static double o_arr2(double A0, double B0, double T) { return A0 * exp(-B0 / T); }
+
+static double o_troe(double k0_300K, double n, double kinf_300K, double m, double T, double M) {
+  double zt = 300.0 / T;
+  double k0 = k0_300K * pow(zt, n) * M;
+  double kinf = kinf_300K * pow(zt, m);
+  double kr = k0 / kinf;
+  return k0 / (1.0 + kr) * pow(0.6, 1.0 / (1.0 + pow(log10(kr), 2)));
+}
+
+static double o_troee(double A, double B, double k0_300K, double n, double kinf_300K, double m,
+                      double T, double M) {
+  return A * exp(-B / T) * o_troe(k0_300K, n, kinf_300K, m, T, M);
+}
+
+static double o_thermal_t2(double c, double d, double T) { return pow(T, 2.0) * c * exp(-d / T); }
+
+static double o_k46(double M, double T) {
+  double k0 = 7.2e-15 * exp(785.0 / T);
+  double k2 = 4.1e-16 * exp(1440.0 / T);
+  double k3 = 1.9e-33 * exp(725.0 / T) * M;
+  return k0 + k3 / (1 + k3 / k2);
+}
+
+/* mechanism tables and the literal rate laws (generated from the JSON) */
+#include "generated/radm2_oracle_tables.h"
+
+void oracle_rconst(double T, double M, const double *J21, double rc_n2o5, double *k) {
+  o_rconst_literal(T, M, J21, rc_n2o5, k);
+}
+
+/* mgpu.cu:552 at relative humidity 0: 1/(3.6e4*exp(-(0/0.28)^2.8 + 300)) */
+double oracle_default_rc_n2o5(void) { return 1.0 / (3.6e4 * exp(-pow(0.0 / 0.28, 2.8) + 300.0)); }
+
+/* ---- tendency (mgpu.cu:1103-1385) ----------------------------------------- */
+static inline double o_factor(int code, const double *V, double h2o, double m) {
+  return code >= 100 ? (code == 100 ? h2o : m) : V[code];
+}
+
+void oracle_fun(const double *V, double h2o, double m, const double *k, double *vdot) {
+  double A[O_NREACT];
+  for (int r = 0; r < O_NREACT; r++) {
+    double a = k[r];
+    for (int q = 0; q < 4 && O_MONO[r][q] >= 0; q++) a = a * o_factor(O_MONO[r][q], V, h2o, m);
+    A[r] = a;
+  }
+  for (int i = 0; i < O_NVAR; i++) {
+    double acc = 0.0;
+    for (int t = O_TEND_START[i]; t < O_TEND_START[i + 1]; t++)
+      acc = fma(O_TEND_C[t], A[O_TEND_R[t]], acc);
+    vdot[i] = acc;
+  }
+}
+
+/* ---- Jacobian in LU layout (mgpu.cu:1685-2651) ---------------------------- */
+static int o_lu_pos(int i, int j) {
+  for (int kk = O_LU_CROW[i]; kk < O_LU_CROW[i + 1]; kk++)
+    if (O_LU_ICOL[kk] == j) return kk;
+  abort();
+}
+
+/* sign = -1 accumulates -J (prepared matrix), +1 accumulates J */
+static void o_jac_acc(const double *V, double h2o, double m, const double *k, double sign,
+                      double *G) {
+  for (int p = 0; p < O_LU_NNZ; p++) G[p] = 0.0;
+  for (int i = 0; i < O_NVAR; i++) {
+    int nt = O_TEND_START[i + 1] - O_TEND_START[i];
+    int nd = 0;
+    for (int d = 0; d < O_NDROP; d++) nd += (O_DROP_I[d] == i);
+    for (int t = 0; t < nt + nd; t++) {
+      int r;
+      double c;
+      if (t < nt) {
+        r = O_TEND_R[O_TEND_START[i] + t];
+        c = O_TEND_C[O_TEND_START[i] + t];
+      } else {
+        int seen = -1, d;
+        for (d = 0; d < O_NDROP; d++)
+          if (O_DROP_I[d] == i && ++seen == t - nt) break;
+        r = O_DROP_R[d];
+        c = O_DROP_C[d];
+      }
+      for (int q = 0; q < 4 && O_MONO[r][q] >= 0; q++) {
+        int code = O_MONO[r][q];
+        if (code >= 100) continue;
+        double term = k[r];
+        for (int q2 = 0; q2 < 4 && O_MONO[r][q2] >= 0; q2++)
+          if (q2 != q) term = term * o_factor(O_MONO[r][q2], V, h2o, m);
+        int p = o_lu_pos(i, code);
+        G[p] = fma(sign * c, term, G[p]);
+      }
+    }
+  }
+}
+
+void oracle_jac(const double *V, double h2o, double m, const double *k, double *jvs) {
+  o_jac_acc(V, h2o, m, k, 1.0, jvs);
+}
+
+void oracle_jac_prep(const double *V, double h2o, double m, const double *k, double ghinv,
+                     double *G) {
+  o_jac_acc(V, h2o, m, k, -1.0, G);
+  for (int i = 0; i < O_NVAR; i++) G[O_LU_DIAG[i]] = G[O_LU_DIAG[i]] + ghinv;
+}
+
+/* ---- LU (mgpu.cu:2740-2779) and solve (mgpu.cu:2873-3090) ----------------- */
+int oracle_decomp(double *G, int ref_division) {
+  double W[O_NVAR];
+  for (int k = 0; k < O_NVAR; k++) {
+    if (G[O_LU_DIAG[k]] == 0.0) return k + 1;
+    for (int kk = O_LU_CROW[k]; kk < O_LU_CROW[k + 1]; kk++) W[O_LU_ICOL[kk]] = G[kk];
+    for (int kk = O_LU_CROW[k]; kk < O_LU_DIAG[k]; kk++) {
+      int j = O_LU_ICOL[kk];
+      double a;
+      if (ref_division) { /* a = -W[j]/U_jj; W[j] = -a; W[i] += a*U_ji */
+        a = -W[j] / G[O_LU_DIAG[j]];
+        W[j] = -a;
+      } else { /* inverse pivot on the diagonal */
+        W[j] = W[j] * G[O_LU_DIAG[j]];
+        a = -W[j];
+      }
+      for (int jj = O_LU_DIAG[j] + 1; jj < O_LU_CROW[j + 1]; jj++)
+        W[O_LU_ICOL[jj]] = ref_division ? W[O_LU_ICOL[jj]] + a * G[jj]
+                                        : fma(a, G[jj], W[O_LU_ICOL[jj]]);
+    }
+    for (int kk = O_LU_CROW[k]; kk < O_LU_CROW[k + 1]; kk++) G[kk] = W[O_LU_ICOL[kk]];
+    if (!ref_division) G[O_LU_DIAG[k]] = 1.0 / W[k];
+  }
+  return 0;
+}
+
+void oracle_solve(const double *LU, double *x, int ref_division) {
+  for (int i = 0; i < O_NVAR; i++)
+    for (int kk = O_LU_CROW[i]; kk < O_LU_DIAG[i]; kk++)
+      x[i] = ref_division ? x[i] - LU[kk] * x[O_LU_ICOL[kk]] : fma(-LU[kk], x[O_LU_ICOL[kk]], x[i]);
+  for (int i = O_NVAR - 1; i >= 0; i--) {
+    double acc = x[i];
+    for (int kk = O_LU_DIAG[i] + 1; kk < O_LU_CROW[i + 1]; kk++)
+      acc = ref_division ? acc - LU[kk] * x[O_LU_ICOL[kk]] : fma(-LU[kk], x[O_LU_ICOL[kk]], acc);
+    x[i] = ref_division ? acc / LU[O_LU_DIAG[i]] : acc * LU[O_LU_DIAG[i]];
+  }
+}
+
+/* ---- options (mgpu.cu:3688-3811, defaults of INTEGRATE mgpu.cu:3850-3873) - */
+int oracle_resolve(double dt, double rtol, double atol, const oracle_opts_in *in,
+                   oracle_opts *o) {
+  oracle_opts_in d;
+  if (!in) { /* INTEGRATE: all zero except Hstart = 0.01*dt, Ros3 */
+    memset(&d, 0, sizeof d);
+    d.hstart = 0.01 * dt;
+    d.method = 2;
+    in = &d;
+  }
+  double span = fabs(dt);
+  o->roundoff = DBL_EPSILON;
+  o->max_steps = in->max_steps == 0 ? 100000 : in->max_steps;
+  if (o->max_steps < 0) return -1;
+  /* method: 0 selects Ros3 (the reference's 0 fails its own range check) */
+  int method = in->method == 0 ? 2 : in->method;
+  if (method < 1 || method > 5) return -2;
+  o->hmin = in->hmin;
+  if (in->hmin < 0.0) return -3;
+  o->hmax = in->hmax == 0.0 ? span : O_MIN(fabs(in->hmax), span);
+  if (in->hmax < 0.0) return -3;
+  o->hstart = in->hstart == 0.0 ? O_MAX(o->hmin, DELTA_MIN) : O_MIN(fabs(in->hstart), span);
+  if (in->hstart < 0.0) return -3;
+  o->facmin = in->facmin == 0.0 ? 0.2 : in->facmin;
+  if (in->facmin < 0.0) return -4;
+  o->facmax = in->facmax == 0.0 ? 6.0 : in->facmax;
+  if (in->facmax < 0.0) return -4;
+  o->facrej = in->facrej == 0.0 ? 0.1 : in->facrej;
+  if (in->facrej < 0.0) return -4;
+  o->facsafe = in->facsafe == 0.0 ? 0.9 : in->facsafe;
+  if (in->facsafe < 0.0) return -4;
+  if (atol <= 0.0 || rtol <= 10.0 * o->roundoff || rtol >= 1.0) return -5;
+  if (method != 2) return -2; /* only Ros3 is implemented (mgpu.cu:3802-3811) */
+  return 0;
+}
+
+/* ---- one cell (CS3: mgpu.cu:3475-3625 with acceptStep 3223-3260) ---------- */
+int oracle_ros3_cell(double *y, const double *k, double h2o, double m, double dt, double rtol,
+                     double atol, const oracle_opts *o, double h0, int32_t *nstep,
+                     int32_t *nacc, int32_t *nrej, double *hexit) {
+  double G[O_LU_NNZ], F0[O_NVAR], F2[O_NVAR], K1[O_NVAR], K2[O_NVAR], K3[O_NVAR];
+  double yn[O_NVAR];
+  const double Tend = dt, rnd = o->roundoff;
+  double T = 0.0;
+  double H = O_MIN(o->hstart, o->hmax);
+  if (fabs(H) <= 10.0 * rnd) H = DELTA_MIN;
+  if (h0 > 0.0) H = O_MIN(O_MAX(h0, O_MAX(o->hmin, DELTA_MIN)), o->hmax);
+  int rejLast = 0, rejMore = 0, status = 1;
+  int32_t ns = 0, na = 0, nr = 0;
+  while ((T - Tend) + rnd <= 0.0) {
+    if (ns > o->max_steps) { status = -6; break; }
+    if (T + 0.1 * H == T || H <= rnd) { status = -7; break; }
+    H = O_MIN(H, fabs(Tend - T));
+    oracle_fun(y, h2o, m, k, F0);
+    for (;;) {
+      const double ghinv = 1.0 / (H * ROS_GAMMA1);
+      oracle_jac_prep(y, h2o, m, k, ghinv, G);
+      if (oracle_decomp(G, 0) != 0) { status = -8; goto done; }
+      for (int i = 0; i < O_NVAR; i++) K1[i] = F0[i];
+      oracle_solve(G, K1, 0);
+      for (int i = 0; i < O_NVAR; i++) yn[i] = y[i] + K1[i];
+      oracle_fun(yn, h2o, m, k, F2);
+      const double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
+      for (int i = 0; i < O_NVAR; i++) K2[i] = fma(K1[i], hc21, F2[i]);
+      oracle_solve(G, K2, 0);
+      for (int i = 0; i < O_NVAR; i++) K3[i] = fma(K2[i], hc32, fma(K1[i], hc31, F2[i]));
+      oracle_solve(G, K3, 0);
+      double err = 0.0;
+      for (int i = 0; i < O_NVAR; i++) {
+        double s = fma(ROS_M3, K3[i], fma(ROS_M2, K2[i], ROS_M1 * K1[i]));
+        yn[i] = y[i] + s;
+        double e = fma(ROS_E3, K3[i], fma(ROS_E2, K2[i], ROS_E1 * K1[i]));
+        double ymax = O_MAX(fabs(y[i]), fabs(yn[i]));
+        double q = e / fma(rtol, ymax, atol);
+        err = fma(q, q, err);
+      }
+      err = sqrt(err / (double)O_NVAR);
+      double fac = O_MIN(o->facmax, O_MAX(o->facmin, o->facsafe / pow(err, 1.0 / ROS_ELO)));
+      double Hnew = H * fac;
+      ns++;
+      if (err <= 1.0 || H <= o->hmin) {
+        for (int i = 0; i < O_NVAR; i++) y[i] = yn[i];
+        T = T + H;
+        if (rejLast) Hnew = O_MIN(Hnew, H);
+        rejLast = 0;
+        rejMore = 0;
+        H = Hnew;
+        na++;
+        break;
+      }
+      if (rejMore) Hnew = H * o->facrej;
+      rejMore = rejLast;
+      rejLast = 1;
+      H = Hnew;
+      nr++;
+    }
+  }
+done:
+  if (nstep) *nstep = ns;
+  if (nacc) *nacc = na;
+  if (nrej) *nrej = nr;
+  if (hexit) *hexit = H;
+  return status;
+}
+
+int oracle_integrate(int64_t n, int64_t ld, double *conc, const double *temp,
+                     const double *cair, const double *h2o, const double *jrate,
+                     const double *rc_n2o5, double dt, double rtol, double atol,
+                     const oracle_opts_in *opts, const double *h0, int32_t *nstep,
+                     int32_t *nacc, int32_t *nrej, int32_t *status, double *hexit,
+                     int nthreads) {
+  oracle_opts o;
+  int rc = oracle_resolve(dt, rtol, atol, opts, &o);
+  if (rc != 0) return rc;
+  const double rc_def = oracle_default_rc_n2o5();
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
+#endif
+  for (int64_t p = 0; p < n; p++) {
+    double y[O_NVAR], k[O_NREACT], J[O_NJ];
+    for (int j = 0; j < O_NJ; j++) J[j] = jrate[j * ld + p];
+    oracle_rconst(temp[p], cair[p], J, rc_n2o5 ? rc_n2o5[p] : rc_def, k);
+    for (int i = 0; i < O_NVAR; i++) y[i] = conc[i * ld + p];
+    int32_t a = 0, b = 0, c = 0;
+    double hx = 0.0;
+    int st = oracle_ros3_cell(y, k, h2o[p], cair[p], dt, rtol, atol, &o, h0 ? h0[p] : 0.0, &a,
+                              &b, &c, &hx);
+    for (int i = 0; i < O_NVAR; i++) conc[i * ld + p] = y[i];
+    if (nstep) nstep[p] = a;
+    if (nacc) nacc[p] = b;
+    if (nrej) nrej[p] = c;
+    if (status) status[p] = st;
+    if (hexit) hexit[p] = hx;
+  }
+  return 0;
+}

@@ -1,0 +1,64 @@
+/*
+ * CPU oracle for the RADM2 Rosenbrock (KPP Ros3) per-cell integrator.
+ *
+ * TEST INFRASTRUCTURE ONLY: the parity checker for the HIP path and the
+ * timed CPU baseline.  No product code links, loads or calls it.
+ *
+ * Layouts match the product C ABI: state arrays are structure-of-arrays with
+ * the cell index fastest, x[s*ld + p].  Per-cell building blocks take one
+ * cell's contiguous vectors.
+ */
+#ifndef RADM2_ORACLE_H
+#define RADM2_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define O_NVAR 59
+#define O_NREACT 156
+#define O_LU_NNZ 659
+#define O_NJ 21
+
+typedef struct {
+  double hmin, hmax, hstart, facmin, facmax, facrej, facsafe;
+  int32_t max_steps, method;
+} oracle_opts_in;
+
+typedef struct {
+  double hmin, hmax, hstart, facmin, facmax, facrej, facsafe, roundoff;
+  int32_t max_steps;
+} oracle_opts;
+
+/* building blocks, one cell */
+void oracle_rconst(double T, double M, const double *J21, double rc_n2o5, double *k);
+double oracle_default_rc_n2o5(void);
+void oracle_fun(const double *V, double h2o, double m, const double *k, double *vdot);
+void oracle_jac(const double *V, double h2o, double m, const double *k, double *jvs);
+void oracle_jac_prep(const double *V, double h2o, double m, const double *k, double ghinv,
+                     double *G);
+int oracle_decomp(double *G, int ref_division);
+void oracle_solve(const double *LU, double *x, int ref_division);
+
+/* option validation and defaults (KPP IERR codes -1..-5, 0 = ok) */
+int oracle_resolve(double dt, double rtol, double atol, const oracle_opts_in *in,
+                   oracle_opts *out);
+
+/* one cell: returns status 1 ok, -6/-7/-8 */
+int oracle_ros3_cell(double *y, const double *k, double h2o, double m, double dt, double rtol,
+                     double atol, const oracle_opts *o, double h0, int32_t *nstep,
+                     int32_t *nacc, int32_t *nrej, double *hexit);
+
+/* many cells, SoA; returns the call-level code (0 or -1..-5) */
+int oracle_integrate(int64_t n, int64_t ld, double *conc, const double *temp,
+                     const double *cair, const double *h2o, const double *jrate,
+                     const double *rc_n2o5, double dt, double rtol, double atol,
+                     const oracle_opts_in *opts, const double *h0, int32_t *nstep,
+                     int32_t *nacc, int32_t *nrej, int32_t *status, double *hexit,
+                     int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

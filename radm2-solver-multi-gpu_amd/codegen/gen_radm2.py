@@ -1,0 +1,377 @@
+#!/usr/bin/env python3
+"""Build-time generator: mechanism table -> straight-line HIP device code.
+
+Reads ``mech/radm2.json`` and writes ``csrc/generated/radm2_gen.h`` holding
+the per-cell numerics of the RADM2 Rosenbrock step as fully unrolled,
+branch-free device functions with compile-time array indices:
+
+  rconst    rate constants (reference RCONST, mgpu.cu:752-919) with the
+            exponentials shared across reactions of equal activation
+            temperature and the Troe powers shared across equal exponents
+  fun       156 mass-action rates -> 59 tendencies (mgpu.cu:1084-1386)
+  jac_prep  G = -J + ghinv*I written straight into the LU layout
+            (mgpu.cu:1672-2652 plus ros_PrepareMatrix, mgpu.cu:3128-3149,
+            with the shift on the true diagonal)
+  decomp    row-wise (IKJ) no-pivot sparse LU in the 659-position pattern,
+            inverse pivots stored on the diagonal (mgpu.cu:2726-2780)
+  solve     forward (unit L) and backward (U) substitution (mgpu.cu:2854-3096)
+
+Arithmetic order contract (shared with the CPU oracle, which implements the
+same order with table-driven loops, so both produce identical bits for the
+same rate constants):
+  * a monomial rate is k[r] times its factors left to right;
+  * every accumulation is acc = fma(c, term, acc) starting from 0, in the
+    order the mechanism table lists the terms;
+  * a Jacobian term is c * (k[r] * the other factors left to right), taken
+    while walking species i, its tendency terms (dropped terms last), and
+    the V factors of each monomial in order; G accumulates -c*term, and the
+    diagonal shift is added after all terms;
+  * decomposition and solves update in increasing column order.
+
+Array arguments are generic "indexables" (raw pointers or strided views),
+so the caller decides whether a working array lives in VGPRs, LDS or a
+global SoA workspace.
+"""
+import ast
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+
+
+def lit(c):
+    """C double literal for a coefficient string such as '0.65' or '2'."""
+    s = c.strip()
+    v = float(s)
+    if v == int(v) and "e" not in s.lower() and "." not in s:
+        return "%d.0" % int(v)
+    return s
+
+
+def load_mech(path=None):
+    path = path or os.path.join(PKG, "mech", "radm2.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+# --------------------------------------------------------------------------
+# rate constants
+# --------------------------------------------------------------------------
+class RateEmitter:
+    """Translate the rate-law notation into C with shared subexpressions."""
+
+    def __init__(self):
+        self.pre = []          # (name, C expression)
+        self.cache = {}
+
+    def _shared(self, key, expr):
+        if key not in self.cache:
+            name = "rc_s%d" % len(self.cache)
+            self.cache[key] = name
+            self.pre.append((name, expr))
+        return self.cache[key]
+
+    def exp_over_T(self, num):
+        # exp(num/T); num is a C literal string (its value decides sharing)
+        return self._shared(("expT", float(num)), "exp(%s / T)" % num)
+
+    def zt_pow(self, n):
+        return self._shared(("ztpow", float(n)), "pow(zt, %s)" % n)
+
+    def emit(self, node):
+        if isinstance(node, ast.Expression):
+            return self.emit(node.body)
+        if isinstance(node, ast.Constant):
+            return self.src_of(node)
+        if isinstance(node, ast.Name):
+            return {"T": "T", "M": "M", "RC_N2O5": "RC"}[node.id]
+        if isinstance(node, ast.Subscript):
+            assert node.value.id == "J"
+            return "J[%d]" % node.slice.value
+        if isinstance(node, ast.UnaryOp) and isinstance(node.op, ast.USub):
+            return "(-%s)" % self.emit(node.operand)
+        if isinstance(node, ast.BinOp):
+            op = {ast.Add: "+", ast.Sub: "-", ast.Mult: "*", ast.Div: "/"}[type(node.op)]
+            return "(%s %s %s)" % (self.emit(node.left), op, self.emit(node.right))
+        if isinstance(node, ast.Call):
+            fn = node.func.id
+            a = [self.src_of(x) for x in node.args]
+            if fn == "ARR2":
+                # A*exp(-B/T): -B/T == (-B)/T exactly, so share by -B
+                return "(%s * %s)" % (a[0], self.exp_over_T(neg(a[1])))
+            if fn == "exp":
+                # exp(c/T) as written
+                arg = node.args[0]
+                assert isinstance(arg, ast.BinOp) and isinstance(arg.op, ast.Div) \
+                    and isinstance(arg.right, ast.Name) and arg.right.id == "T"
+                return self.exp_over_T(self.src_of(arg.left))
+            if fn == "pow":
+                return "pow(%s, %s)" % (self.emit(node.args[0]), self.emit(node.args[1]))
+            if fn == "TROE":
+                k0, n, ki, m = a
+                return "radm2_troe(%s, %s, %s, %s, M)" % (k0, self.zt_pow(n), ki, self.zt_pow(m))
+            if fn == "TROEE":
+                A, B, k0, n, ki, m = a
+                return "(%s * %s * radm2_troe(%s, %s, %s, %s, M))" % (
+                    A, self.exp_over_T(neg(B)), k0, self.zt_pow(n), ki, self.zt_pow(m))
+            if fn == "THERMAL_T2":
+                c, d = a
+                return "((T * T) * %s * %s)" % (c, self.exp_over_T(neg(d)))
+            if fn == "K46":
+                return "radm2_k46(T, M, %s, %s, %s)" % (
+                    self.exp_over_T("785.0"), self.exp_over_T("1440.0"), self.exp_over_T("725.0"))
+            raise ValueError(fn)
+        raise ValueError(ast.dump(node))
+
+    def src_of(self, node):
+        if isinstance(node, ast.Constant):
+            return self._src[node.col_offset:node.end_col_offset]
+        if isinstance(node, ast.UnaryOp) and isinstance(node.op, ast.USub) \
+                and isinstance(node.operand, ast.Constant):
+            return "-" + self.src_of(node.operand)
+        return self.emit(node)
+
+
+def neg(s):
+    s = s.strip()
+    return s[1:] if s.startswith("-") else "-" + s
+
+
+def gen_rconst(mech):
+    em = RateEmitter()
+    body = []
+    for r, e in enumerate(mech["rates"]):
+        em._src = e
+        tree = ast.parse(e, mode="eval")
+        body.append("  k[%d] = %s;" % (r, em.emit(tree)))
+    out = []
+    out.append("// rate constants: %d reactions, %d shared exp/pow subexpressions"
+               % (len(mech["rates"]), len(em.pre)))
+    out.append("template <class KA>")
+    out.append("__device__ __forceinline__ void rconst(double T, double M, const double* __restrict__ J,"
+               " int64_t sJ, double RC, KA k) {")
+    out.append("  (void)RC;")
+    out.append("  const double zt = 300.0 / T;")
+    # J slots are addressed with a stride
+    for name, ex in em.pre:
+        out.append("  const double %s = %s;" % (name, ex))
+    for ln in body:
+        out.append(ln.replace("J[", "JJ("))
+    out.append("}")
+    text = "\n".join(out)
+    # JJ(i] -> J[i*sJ]
+    import re
+    text = re.sub(r"JJ\((\d+)\]", r"J[\1 * sJ]", text)
+    return text, len(em.pre)
+
+
+# --------------------------------------------------------------------------
+# tendency
+# --------------------------------------------------------------------------
+def factor_src(kind, i):
+    return "v[%d]" % i if kind == "V" else ("h2o" if i == 0 else "m")
+
+
+def gen_fun(mech):
+    out = ["// tendency: %d rates, %d terms" % (mech["nreact"], sum(len(t) for t in mech["tendency"]))]
+    out.append("template <class KA, class VA, class FA>")
+    out.append("__device__ __forceinline__ void fun(KA k, VA v, double h2o, double m, FA f) {")
+    for r, facs in enumerate(mech["monomials"]):
+        expr = "k[%d]" % r
+        for kind, i in facs:
+            expr = "(%s * %s)" % (expr, factor_src(kind, i))
+        out.append("  const double a%d = %s;" % (r, expr))
+    for i, terms in enumerate(mech["tendency"]):
+        acc = None
+        for r, c in terms:
+            cv = float(c)
+            if acc is None:
+                acc = ("a%d" % r) if cv == 1 else ("(-a%d)" % r if cv == -1 else "(%s * a%d)" % (lit(c), r))
+            elif cv == 1:
+                acc = "(%s + a%d)" % (acc, r)
+            elif cv == -1:
+                acc = "(%s - a%d)" % (acc, r)
+            else:
+                acc = "fma(%s, a%d, %s)" % (lit(c), r, acc)
+        out.append("  f[%d] = %s;" % (i, acc if acc else "0.0"))
+    out.append("}")
+    return "\n".join(out)
+
+
+# --------------------------------------------------------------------------
+# Jacobian in LU layout, negated, diagonal shifted
+# --------------------------------------------------------------------------
+def jac_terms(mech):
+    """Per LU position: list of (coef string, r, [other factors]) in the
+    contract order (species i, its tendency terms then dropped terms,
+    V factors of the monomial in order)."""
+    lu = mech["lu"]
+    pos = {}
+    for row in range(59):
+        for kk in range(lu["crow"][row], lu["crow"][row + 1]):
+            pos[(row, lu["icol"][kk])] = kk
+    terms = [[] for _ in range(659)]
+    for i in range(59):
+        full = list(mech["tendency"][i]) + [[r, c] for (ii, r, c) in mech["fun_dropped_terms"] if ii == i]
+        for r, c in full:
+            facs = mech["monomials"][r]
+            for q, (kind, j) in enumerate(facs):
+                if kind != "V":
+                    continue
+                others = [f for q2, f in enumerate(facs) if q2 != q]
+                terms[pos[(i, j)]].append((c, r, others))
+    return terms
+
+
+def gen_jac_prep(mech):
+    terms = jac_terms(mech)
+    lu = mech["lu"]
+    diag = set(lu["diag"])
+    nt = sum(len(t) for t in terms)
+    out = ["// G = -J + ghinv*I in LU layout: %d Jacobian terms over %d positions" % (nt, 659)]
+    out.append("template <class KA, class VA, class GA>")
+    out.append("__device__ __forceinline__ void jac_prep(KA k, VA v, double h2o, double m,"
+               " double ghinv, GA g) {")
+    for p in range(659):
+        acc = None
+        for c, r, others in terms[p]:
+            t = "k[%d]" % r
+            for kind, j in others:
+                t = "(%s * %s)" % (t, factor_src(kind, j))
+            cv = float(c)
+            if acc is None:
+                acc = ("(-%s)" % t) if cv == 1 else (t if cv == -1 else "(%s * %s)" % (lit(neg(c)), t))
+            elif cv == 1:
+                acc = "(%s - %s)" % (acc, t)
+            elif cv == -1:
+                acc = "(%s + %s)" % (acc, t)
+            else:
+                acc = "fma(%s, %s, %s)" % (lit(neg(c)), t, acc)
+        if p in diag:
+            acc = "(%s + ghinv)" % acc if acc else "ghinv"
+        out.append("  g[%d] = %s;" % (p, acc if acc else "0.0"))
+    out.append("}")
+    return "\n".join(out)
+
+
+# --------------------------------------------------------------------------
+# LU decomposition and solve
+# --------------------------------------------------------------------------
+def gen_decomp(mech):
+    lu = mech["lu"]
+    crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
+    out = ["// row-wise no-pivot LU; diagonal holds 1/U_kk on return."]
+    out.append("// Returns 0, or k+1 for the first row whose prepared diagonal is 0.")
+    out.append("template <class GA>")
+    out.append("__device__ __forceinline__ int decomp(GA g) {")
+    out.append("  int sing = 0;")
+    nfma = 0
+    for k in range(59):
+        cols = icol[crow[k]:crow[k + 1]]
+        out.append("  { // row %d" % k)
+        out.append("    sing = (sing == 0 && g[%d] == 0.0) ? %d : sing;" % (diag[k], k + 1))
+        for kk in range(crow[k], crow[k + 1]):
+            out.append("    double w%d = g[%d];" % (icol[kk], kk))
+        for kk in range(crow[k], diag[k]):
+            j = icol[kk]
+            out.append("    w%d = w%d * g[%d];" % (j, j, diag[j]))
+            for jj in range(diag[j] + 1, crow[j + 1]):
+                c = icol[jj]
+                assert c in cols
+                out.append("    w%d = fma(-w%d, g[%d], w%d);" % (c, j, jj, c))
+                nfma += 1
+        for kk in range(crow[k], crow[k + 1]):
+            c = icol[kk]
+            if kk == diag[k]:
+                out.append("    g[%d] = 1.0 / w%d;" % (kk, c))
+            else:
+                out.append("    g[%d] = w%d;" % (kk, c))
+        out.append("  }")
+    out.append("  return sing;")
+    out.append("}")
+    return "\n".join(out), nfma
+
+
+def gen_solve(mech):
+    lu = mech["lu"]
+    crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
+    out = ["// x <- G^-1 x with unit-lower L and U (inverse pivots on the diagonal)"]
+    out.append("template <class GA, class XA>")
+    out.append("__device__ __forceinline__ void solve(GA g, XA x) {")
+    for i in range(59):
+        if diag[i] > crow[i]:
+            acc = "x[%d]" % i
+            for kk in range(crow[i], diag[i]):
+                acc = "fma(-g[%d], x[%d], %s)" % (kk, icol[kk], acc)
+            out.append("  x[%d] = %s;" % (i, acc))
+    for i in range(58, -1, -1):
+        acc = "x[%d]" % i
+        for kk in range(diag[i] + 1, crow[i + 1]):
+            acc = "fma(-g[%d], x[%d], %s)" % (kk, icol[kk], acc)
+        out.append("  x[%d] = %s * g[%d];" % (i, acc, diag[i]))
+    out.append("}")
+    return "\n".join(out)
+
+
+def generate(dst=None, mech_path=None):
+    mech = load_mech(mech_path)
+    dst = dst or os.path.join(PKG, "csrc", "generated", "radm2_gen.h")
+    os.makedirs(os.path.dirname(dst), exist_ok=True)
+    rc, nshared = gen_rconst(mech)
+    dec, nfma = gen_decomp(mech)
+    parts = [
+        "// GENERATED by codegen/gen_radm2.py from mech/radm2.json -- do not edit.",
+        "#pragma once",
+        "#include <hip/hip_runtime.h>",
+        "#include <stdint.h>",
+        "namespace radm2 {",
+        "namespace gen {",
+        "constexpr int NVAR = %d;" % mech["nvar"],
+        "constexpr int NREACT = %d;" % mech["nreact"],
+        "constexpr int LU_NNZ = %d;" % mech["lu"]["nnz"],
+        "constexpr int NJ = 21;",
+        "constexpr int DECOMP_FMA = %d;" % nfma,
+        "constexpr int LU_DIAG[%d] = {%s};" % (mech["nvar"], ",".join(map(str, mech["lu"]["diag"]))),
+        "",
+        "// Troe fall-off with Fc = 0.6 (Stockwell et al. 1997), powers of 300/T precomputed",
+        "__device__ __forceinline__ double radm2_troe(double k0_300, double ztn, double kinf_300,"
+        " double ztm, double M) {",
+        "  const double k0 = k0_300 * ztn * M;",
+        "  const double kinf = kinf_300 * ztm;",
+        "  const double kr = k0 / kinf;",
+        "  const double lg = log10(kr);",
+        "  return k0 / (1.0 + kr) * pow(0.6, 1.0 / (1.0 + lg * lg));",
+        "}",
+        "// HNO3 + OH three-channel rate: k0 + k3/(1 + k3/k2)",
+        "__device__ __forceinline__ double radm2_k46(double T, double M, double e785,"
+        " double e1440, double e725) {",
+        "  (void)T;",
+        "  const double k0 = 7.2e-15 * e785;",
+        "  const double k2 = 4.1e-16 * e1440;",
+        "  const double k3 = 1.9e-33 * e725 * M;",
+        "  return k0 + k3 / (1.0 + k3 / k2);",
+        "}",
+        "",
+        rc,
+        "",
+        gen_fun(mech),
+        "",
+        gen_jac_prep(mech),
+        "",
+        dec,
+        "",
+        gen_solve(mech),
+        "",
+        "}  // namespace gen",
+        "}  // namespace radm2",
+        "",
+    ]
+    with open(dst, "w") as f:
+        f.write("\n".join(parts))
+    return dst
+
+
+if __name__ == "__main__":
+    print(generate(*(sys.argv[1:2] or [None])))

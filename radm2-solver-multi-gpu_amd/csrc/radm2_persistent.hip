@@ -1,0 +1,107 @@
+// Persistent work-stealing kernel: a fixed grid of resident waves; each lane
+// integrates one cell at a time and, when its cell finishes, takes the next
+// cell index from a global counter.  Refills are wave-aggregated (one atomic
+// per refill round) and happen once at least `refill_min` lanes of the wave
+// are idle, trading idle lanes against the cost of the fused rate-constant
+// evaluation that a refill runs.  Cells of uneven stiffness (night cells
+// finish in a few steps, plume cells take tens) therefore no longer hold a
+// whole wave hostage to its slowest lane.
+#include "radm2_cell_io.h"
+
+namespace radm2 {
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_ros3_persistent(LaunchArgs a) {
+  double kk[NREACT], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
+  double h2o = 0.0, m = 0.0;
+  CellState s;
+  s.status = ST_OK;
+  s.newstep = 0;
+  int64_t cell = -1;  // global work index held by this lane, -1 when none
+  int di = 0;
+  bool exhausted = false;  // wave-uniform: the counter has passed the end
+  const int lane = threadIdx.x & 63;
+  unsigned long long refills = 0;
+  for (;;) {
+    const bool idle = (s.status != ST_RUNNING);
+    const unsigned long long idle_mask = __ballot(idle);
+    const int nidle = __popcll(idle_mask);
+    if (!exhausted && nidle > 0 && (nidle >= a.refill_min || nidle == 64)) {
+      if (idle && cell >= 0) {
+        const DomainDesc& d = a.doms[di];
+        cell_store(d, cell - d.first, y, s, s.hfirst, a.stats);
+        cell = -1;
+      }
+      const int leader = __ffsll((long long)idle_mask) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(a.counter, (unsigned long long)nidle);
+      base = __shfl(base, leader);
+      const int rank = __popcll(idle_mask & ((1ull << lane) - 1ull));
+      if (idle) {
+        const int64_t gi = (int64_t)base + rank;
+        if (gi < a.total) {
+          di = find_domain(a.doms, a.ndom, gi);
+          const DomainDesc& d = a.doms[di];
+          cell = gi;
+          cell_load(d, gi - d.first, kk, y, h2o, m, s);
+        }
+      }
+      if ((int64_t)(base + (unsigned long long)nidle) >= a.total) exhausted = true;
+      refills++;
+    }
+    if (__ballot(s.status == ST_RUNNING) == 0ull) {
+      if (exhausted) break;
+      continue;  // all idle but work remains: the next round refills all 64
+    }
+    if (s.status == ST_RUNNING) {
+      const Ros3Opts& o = a.doms[di].opts;
+      if (s.newstep) cell_step_top(s, o);
+      if (s.status == ST_RUNNING) ros3_attempt(s, o, kk, y, h2o, m, lu, f, k1, k2, k3);
+    }
+  }
+  if (cell >= 0) {
+    const DomainDesc& d = a.doms[di];
+    cell_store(d, cell - d.first, y, s, s.hfirst, a.stats);
+  }
+  if (a.stats && lane == 0) atomicAdd(&a.stats[3], refills);
+}
+
+template <int BLOCK>
+static hipError_t launch_p(int blocks, const LaunchArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(k_ros3_persistent<BLOCK>, dim3(blocks), dim3(BLOCK), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_persistent(int block, int blocks, const LaunchArgs& a, hipStream_t st) {
+  if (a.total <= 0) return hipSuccess;
+  switch (block) {
+    case 64: return launch_p<64>(blocks, a, st);
+    case 128: return launch_p<128>(blocks, a, st);
+    case 256: return launch_p<256>(blocks, a, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int persistent_blocks_per_cu(int block) {
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (block) {
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_persistent<64>, 64, 0); break;
+    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_persistent<128>, 128, 0); break;
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_persistent<256>, 256, 0); break;
+    default: break;
+  }
+  return e == hipSuccess ? nb : -1;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_u64, dim3((n + 63) / 64), dim3(64), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace radm2

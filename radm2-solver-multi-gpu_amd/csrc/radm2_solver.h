@@ -1,0 +1,171 @@
+// Per-cell FP64 Rosenbrock (KPP Ros3) step for RADM2, one cell per lane.
+//
+// Device-side building blocks shared by the one-shot grid kernel and the
+// persistent work-stealing kernel.  The mechanism numerics (rate constants,
+// tendency, prepared Jacobian, unrolled LU and solve) come from the
+// generated header; this file holds the Ros3 stage combination, the error
+// norm and the accept/reject state machine.
+//
+// Reference semantics (mgpu.cu of the reference):
+//   Ros3 coefficients                         mgpu.cu:1008-1062
+//   stage formulas (commented stage loop)     mgpu.cu:3595-3625, WAXPY_HC 1540-1552
+//   prepared matrix -Jac0 + 1/(H*gamma1)      mgpu.cu:3128-3149
+//   combination / error norm / step control   mgpu.cu:3223-3260
+//   time-loop mask and H clamp                mgpu.cu:1608-1613
+//   initial step                              mgpu.cu:3475-3477
+// The arithmetic order matches the CPU oracle's (explicit fma, no
+// contraction), so identical rate constants give identical bits.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "generated/radm2_gen.h"
+
+namespace radm2 {
+
+using gen::LU_NNZ;
+using gen::NJ;
+using gen::NREACT;
+using gen::NVAR;
+
+// Ros3 (Sandu et al. 1997 as coded in KPP)
+constexpr double ROS_C21 = -1.0156171083877702091975600115545;
+constexpr double ROS_C31 = 4.0759956452537699824805835358067;
+constexpr double ROS_C32 = 9.2076794298330791242156818474003;
+constexpr double ROS_M1 = 1.0;
+constexpr double ROS_M2 = 6.1697947043828245592553615689730;
+constexpr double ROS_M3 = -0.4277225654321857332623837380651;
+constexpr double ROS_E1 = 0.5;
+constexpr double ROS_E2 = -2.9079558716805469821718236208017;
+constexpr double ROS_E3 = 0.2235406989781156962736090927619;
+constexpr double ROS_GAMMA1 = 0.43586652150845899941601945119356;
+constexpr double ROS_ELO = 3.0;
+constexpr double DELTA_MIN = 1.0e-6;
+
+// Per-cell status codes (KPP IERR)
+constexpr int ST_RUNNING = 0;
+constexpr int ST_OK = 1;
+constexpr int ST_TOO_MANY_STEPS = -6;
+constexpr int ST_STEP_TOO_SMALL = -7;
+constexpr int ST_SINGULAR = -8;
+
+// Resolved integration options for one domain (validated on the host).
+struct Ros3Opts {
+  double dt, rtol, atol;
+  double hmin, hmax, hstart, facmin, facmax, facrej, facsafe, roundoff;
+  int32_t max_steps;
+  int32_t pad;
+};
+
+__device__ __forceinline__ double dmax_ref(double a, double b) { return a >= b ? a : b; }
+__device__ __forceinline__ double dmin_ref(double b, double c) { return b < c ? b : c; }
+
+// Scalar state of one cell between attempts.
+struct CellState {
+  double T, H;
+  double hfirst;    // step-size proposal after the first accepted step
+  int32_t ns, na, nr;
+  int32_t status;   // ST_RUNNING while integrating
+  int32_t rejLast, rejMore;
+  int32_t newstep;  // 1 when the next attempt starts a new time step
+};
+
+__device__ __forceinline__ void cell_begin(CellState& s, const Ros3Opts& o, double h0) {
+  s.T = 0.0;
+  s.hfirst = 0.0;
+  double H = dmin_ref(o.hstart, o.hmax);
+  if (fabs(H) <= 10.0 * o.roundoff) H = DELTA_MIN;
+  if (h0 > 0.0) H = dmin_ref(dmax_ref(h0, dmax_ref(o.hmin, DELTA_MIN)), o.hmax);
+  s.H = H;
+  s.ns = s.na = s.nr = 0;
+  s.status = ST_RUNNING;
+  s.rejLast = s.rejMore = 0;
+  s.newstep = 1;
+}
+
+// Top of a time step: end test, KPP -6/-7 checks, H clamp to the interval.
+__device__ __forceinline__ void cell_step_top(CellState& s, const Ros3Opts& o) {
+  const double Tend = o.dt;
+  if (!((s.T - Tend) + o.roundoff <= 0.0)) {
+    s.status = ST_OK;
+    return;
+  }
+  if (s.ns > o.max_steps) {
+    s.status = ST_TOO_MANY_STEPS;
+    return;
+  }
+  if (s.T + 0.1 * s.H == s.T || s.H <= o.roundoff) {
+    s.status = ST_STEP_TOO_SMALL;
+    return;
+  }
+  s.H = dmin_ref(s.H, fabs(Tend - s.T));
+  s.newstep = 0;
+}
+
+// One Ros3 attempt for a running cell whose step top has been applied.
+//   y   state (updated on acceptance)     k   rate constants
+//   g   659-entry LU workspace            f, k1, k2, k3: 59-entry vectors
+template <class KA, class YA, class GA, class VA>
+__device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA k, YA y,
+                                             double h2o, double m, GA g, VA f, VA k1, VA k2,
+                                             VA k3) {
+  const double H = s.H;
+  // F0 = f(y); K1 = G^-1 F0
+  gen::fun(k, y, h2o, m, k1);
+  const double ghinv = 1.0 / (H * ROS_GAMMA1);
+  gen::jac_prep(k, y, h2o, m, ghinv, g);
+  if (gen::decomp(g) != 0) {
+    s.status = ST_SINGULAR;
+    return;
+  }
+  gen::solve(g, k1);
+  // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
+#pragma unroll
+  for (int i = 0; i < NVAR; i++) k2[i] = y[i] + k1[i];
+  gen::fun(k, k2, h2o, m, f);
+  const double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
+#pragma unroll
+  for (int i = 0; i < NVAR; i++) k2[i] = fma(k1[i], hc21, f[i]);
+  gen::solve(g, k2);
+  // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
+#pragma unroll
+  for (int i = 0; i < NVAR; i++) k3[i] = fma(k2[i], hc32, fma(k1[i], hc31, f[i]));
+  gen::solve(g, k3);
+  // combination and WRMS error norm
+  double err = 0.0;
+#pragma unroll
+  for (int i = 0; i < NVAR; i++) {
+    const double sm = fma(ROS_M3, k3[i], fma(ROS_M2, k2[i], ROS_M1 * k1[i]));
+    const double yn = y[i] + sm;
+    const double e = fma(ROS_E3, k3[i], fma(ROS_E2, k2[i], ROS_E1 * k1[i]));
+    const double ymax = dmax_ref(fabs(y[i]), fabs(yn));
+    const double q = e / fma(o.rtol, ymax, o.atol);
+    err = fma(q, q, err);
+    f[i] = yn;  // keep Ynew until the decision
+  }
+  err = sqrt(err / (double)NVAR);
+  const double fsafe = dmax_ref(o.facmin, o.facsafe / pow(err, 1.0 / ROS_ELO));
+  const double fac = dmin_ref(o.facmax, fsafe);
+  double Hnew = H * fac;
+  s.ns++;
+  if (err <= 1.0 || H <= o.hmin) {
+#pragma unroll
+    for (int i = 0; i < NVAR; i++) y[i] = f[i];
+    s.T = s.T + H;
+    if (s.na == 0) s.hfirst = H * fsafe;  // uncapped proposal (learned-h0 target)
+    if (s.rejLast) Hnew = dmin_ref(Hnew, H);
+    s.rejLast = 0;
+    s.rejMore = 0;
+    s.H = Hnew;
+    s.na++;
+    s.newstep = 1;
+  } else {
+    if (s.rejMore) Hnew = H * o.facrej;
+    s.rejMore = s.rejLast;
+    s.rejLast = 1;
+    s.H = Hnew;
+    s.nr++;
+  }
+}
+
+}  // namespace radm2

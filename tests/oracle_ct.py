@@ -1,0 +1,144 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle_radm2.so) for tests.
+
+The oracle is test infrastructure: only tests/, the smoke check and the
+CPU-baseline leg of bench.py use it, always as the checker.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "liboracle_radm2.so")
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+
+
+class OptsIn(ctypes.Structure):
+    _fields_ = [("hmin", ctypes.c_double), ("hmax", ctypes.c_double),
+                ("hstart", ctypes.c_double), ("facmin", ctypes.c_double),
+                ("facmax", ctypes.c_double), ("facrej", ctypes.c_double),
+                ("facsafe", ctypes.c_double), ("max_steps", ctypes.c_int32),
+                ("method", ctypes.c_int32)]
+
+
+class OptsOut(ctypes.Structure):
+    _fields_ = [("hmin", ctypes.c_double), ("hmax", ctypes.c_double),
+                ("hstart", ctypes.c_double), ("facmin", ctypes.c_double),
+                ("facmax", ctypes.c_double), ("facrej", ctypes.c_double),
+                ("facsafe", ctypes.c_double), ("roundoff", ctypes.c_double),
+                ("max_steps", ctypes.c_int32)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.oracle_rconst.argtypes = [ctypes.c_double, ctypes.c_double, _D, ctypes.c_double, _D]
+        L.oracle_default_rc_n2o5.restype = ctypes.c_double
+        L.oracle_fun.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D, _D]
+        L.oracle_jac.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D, _D]
+        L.oracle_jac_prep.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D, ctypes.c_double, _D]
+        L.oracle_decomp.argtypes = [_D, ctypes.c_int]
+        L.oracle_decomp.restype = ctypes.c_int
+        L.oracle_solve.argtypes = [_D, _D, ctypes.c_int]
+        L.oracle_resolve.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.POINTER(OptsIn), ctypes.POINTER(OptsOut)]
+        L.oracle_resolve.restype = ctypes.c_int
+        L.oracle_ros3_cell.argtypes = [_D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.POINTER(OptsOut),
+                                       ctypes.c_double, _I, _I, _I, _D]
+        L.oracle_ros3_cell.restype = ctypes.c_int
+        L.oracle_integrate.argtypes = [ctypes.c_int64, ctypes.c_int64, _D, _D, _D, _D, _D, _D,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       ctypes.POINTER(OptsIn), _D, _I, _I, _I, _I, _D,
+                                       ctypes.c_int]
+        L.oracle_integrate.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_D)
+    assert a.dtype == np.int32
+    return a.ctypes.data_as(_I)
+
+
+def rconst(T, M, J, rc):
+    k = np.zeros(156)
+    lib().oracle_rconst(T, M, _p(np.ascontiguousarray(J, dtype=np.float64)), rc, _p(k))
+    return k
+
+
+def fun(V, h2o, m, k):
+    out = np.zeros(59)
+    lib().oracle_fun(_p(np.ascontiguousarray(V)), h2o, m, _p(np.ascontiguousarray(k)), _p(out))
+    return out
+
+
+def jac(V, h2o, m, k):
+    out = np.zeros(659)
+    lib().oracle_jac(_p(np.ascontiguousarray(V)), h2o, m, _p(np.ascontiguousarray(k)), _p(out))
+    return out
+
+
+def jac_prep(V, h2o, m, k, ghinv):
+    out = np.zeros(659)
+    lib().oracle_jac_prep(_p(np.ascontiguousarray(V)), h2o, m, _p(np.ascontiguousarray(k)),
+                          ghinv, _p(out))
+    return out
+
+
+def decomp(G, ref_division=False):
+    G = np.array(G, dtype=np.float64)
+    rc = lib().oracle_decomp(_p(G), 1 if ref_division else 0)
+    return rc, G
+
+
+def solve(LU, b, ref_division=False):
+    x = np.array(b, dtype=np.float64)
+    lib().oracle_solve(_p(np.ascontiguousarray(LU)), _p(x), 1 if ref_division else 0)
+    return x
+
+
+def resolve(dt, rtol, atol, opts=None):
+    out = OptsOut()
+    rc = lib().oracle_resolve(dt, rtol, atol, ctypes.byref(opts) if opts else None, ctypes.byref(out))
+    return rc, out
+
+
+def integrate(cells, dt, rtol, atol, opts=None, h0=None, nthreads=0, n=None, use_rc=True):
+    """Integrate make_cells()-style host arrays; returns (conc, counters dict, code)."""
+    conc = np.array(cells["conc"], dtype=np.float64, order="C")
+    ld = conc.shape[1]
+    n = ld if n is None else n
+    ns = np.zeros(ld, np.int32)
+    na = np.zeros(ld, np.int32)
+    nr = np.zeros(ld, np.int32)
+    st = np.zeros(ld, np.int32)
+    hx = np.zeros(ld)
+    rc_arr = np.ascontiguousarray(cells["rc_n2o5"]) if use_rc else None
+    h0a = None if h0 is None else np.ascontiguousarray(h0, dtype=np.float64)
+    code = lib().oracle_integrate(
+        n, ld, _p(conc), _p(np.ascontiguousarray(cells["temp"])),
+        _p(np.ascontiguousarray(cells["cair"])), _p(np.ascontiguousarray(cells["h2o"])),
+        _p(np.ascontiguousarray(cells["jrate"])), _p(rc_arr), dt, rtol, atol,
+        ctypes.byref(opts) if opts else None, _p(h0a), _p(ns), _p(na), _p(nr), _p(st), _p(hx),
+        nthreads)
+    return conc, dict(nstep=ns, nacc=na, nrej=nr, status=st, hexit=hx), code
