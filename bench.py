@@ -2,8 +2,8 @@
 """Benchmark: FP64 RADM2 cell-integrations/s on one MI355X.
 
 Workload: 1,048,576 synthetic cells (128x128x64), dt = 600 s, rtol 1e-3,
-atol 1 molec/cm3, seed 0, zenith U(0,100) deg (about 20% night cells), 5%
-plume cells (NOx 20-500 ppb, VOC x10).  A "step" is one integration of every
+atol 1 molec/cm3, seed 0, zenith U(0,100) deg (10% night cells), 5% plume
+cells (NOx 20-500 ppb, VOC x10).  A "step" is one integration of every
 cell over dt (one cell-integration per cell), inputs resident in HBM.
 
 Paths timed: the one-shot grid kernel (one lane per cell for the whole

@@ -98,9 +98,22 @@ int radm2_create(radm2_ctx **ctx, int device, void *stream);
 void radm2_destroy(radm2_ctx *ctx);
 
 /* Schedule for later calls: RADM2_SCHED_GRID or RADM2_SCHED_PERSISTENT,
- * block size (64, 128 or 256; 0 = tuned default), persistent refill
- * threshold in idle lanes per wave (1..64; 0 = tuned default). */
+ * block size (64 or 256; 0 = tuned default), persistent refill threshold in
+ * idle lanes per wave (1..64; 0 = tuned default).  No reference counterpart
+ * (its launch shapes are hard-coded, e.g. mgpu.cu:1402-1404, 2797). */
 int radm2_set_schedule(radm2_ctx *ctx, int schedule, int block, int refill_min);
+
+/* Kernel variant for later calls (the autotuner's placement/occupancy axes):
+ * placement of the lane-private working arrays -- RADM2_PLACE_PRIVATE
+ * (compiler-managed registers, spills to scratch) or RADM2_PLACE_WORKSPACE
+ * (an explicit wave-interleaved HBM workspace, streamed phase by phase);
+ * -1 = tuned default -- and the register budget as the minimum resident
+ * waves per SIMD (1, 2 or 4; 0 = tuned default).  Returns
+ * RADM2_E_UNSUPPORTED for a combination that is not compiled in.  No
+ * reference counterpart. */
+#define RADM2_PLACE_PRIVATE 0
+#define RADM2_PLACE_WORKSPACE 1
+int radm2_set_kernel(radm2_ctx *ctx, int placement, int waves_per_simd);
 
 /*
  * Integrate ncells cells over [0, dt] with rtol/atol for every species.

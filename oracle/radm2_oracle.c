@@ -253,11 +253,14 @@ int oracle_ros3_cell(double *y, const double *k, double h2o, double m, double dt
   int rejLast = 0, rejMore = 0, status = 1;
   int32_t ns = 0, na = 0, nr = 0;
   while ((T - Tend) + rnd <= 0.0) {
-    if (ns > o->max_steps) { status = -6; break; }
-    if (T + 0.1 * H == T || H <= rnd) { status = -7; break; }
-    H = O_MIN(H, fabs(Tend - T));
+    int first = 1;
     oracle_fun(y, h2o, m, k, F0);
     for (;;) {
+      /* KPP's -6/-7 checks, before every attempt (not only at the top of a
+       * time step) so that endless rejections end; !(H > rnd) catches NaN */
+      if (ns > o->max_steps) { status = -6; goto done; }
+      if (T + 0.1 * H == T || !(H > rnd)) { status = -7; goto done; }
+      if (first) { H = O_MIN(H, fabs(Tend - T)); first = 0; }
       const double ghinv = 1.0 / (H * ROS_GAMMA1);
       oracle_jac_prep(y, h2o, m, k, ghinv, G);
       if (oracle_decomp(G, 0) != 0) { status = -8; goto done; }

@@ -145,7 +145,7 @@ def gen_rconst(mech):
     for r, e in enumerate(mech["rates"]):
         em._src = e
         tree = ast.parse(e, mode="eval")
-        body.append("  k[%d] = %s;" % (r, em.emit(tree)))
+        body.append("  WR(k, %d, %s);" % (r, em.emit(tree)))
     out = []
     out.append("// rate constants: %d reactions, %d shared exp/pow subexpressions"
                % (len(mech["rates"]), len(em.pre)))
@@ -171,33 +171,62 @@ def gen_rconst(mech):
 # tendency
 # --------------------------------------------------------------------------
 def factor_src(kind, i):
-    return "v[%d]" % i if kind == "V" else ("h2o" if i == 0 else "m")
+    return "RD(v, %d)" % i if kind == "V" else ("h2o" if i == 0 else "m")
 
 
 def gen_fun(mech):
-    out = ["// tendency: %d rates, %d terms" % (mech["nreact"], sum(len(t) for t in mech["tendency"]))]
-    out.append("template <class KA, class VA, class FA>")
+    """Tendency in reaction-major (scatter) order: each rate is formed once
+    and added into the accumulators of the species it moves, so only the 59
+    accumulators stay live.  Every species' terms are listed in increasing
+    reaction order, so each accumulator sums in exactly the table order.
+    RF > 0 launders the input views every RF reactions."""
+    nterms = sum(len(t) for t in mech["tendency"])
+    out = ["// tendency: %d rates, %d terms (reaction-major scatter)" % (mech["nreact"], nterms)]
+    out.append("template <int RF, class KA, class VA, class FA>")
     out.append("__device__ __forceinline__ void fun(KA k, VA v, double h2o, double m, FA f) {")
+    by_r = [[] for _ in range(mech["nreact"])]
+    for i, terms in enumerate(mech["tendency"]):
+        for r, c in terms:
+            by_r[r].append((i, c))
+    started = set()
+    for i in range(59):
+        if not mech["tendency"][i]:
+            out.append("  double f%d = 0.0;" % i)
+            started.add(i)
     for r, facs in enumerate(mech["monomials"]):
-        expr = "k[%d]" % r
+        if r > 0:
+            out.append("  if (RF > 0 && %d %% (RF > 0 ? RF : 1) == 0) { launder(k); launder(v); }" % r)
+        if not by_r[r]:
+            continue
+        expr = "RD(k, %d)" % r
         for kind, i in facs:
             expr = "(%s * %s)" % (expr, factor_src(kind, i))
-        out.append("  const double a%d = %s;" % (r, expr))
-    for i, terms in enumerate(mech["tendency"]):
-        acc = None
-        for r, c in terms:
+        out.append("  { const double a = %s;" % expr)
+        for i, c in by_r[r]:
             cv = float(c)
-            if acc is None:
-                acc = ("a%d" % r) if cv == 1 else ("(-a%d)" % r if cv == -1 else "(%s * a%d)" % (lit(c), r))
+            if i not in started:
+                rhs = "a" if cv == 1 else ("(-a)" if cv == -1 else "(%s * a)" % lit(c))
+                out.append("    double f%d = %s;" % (i, rhs) if False else "    f%d_init = %s;" % (i, rhs))
+                started.add(i)
             elif cv == 1:
-                acc = "(%s + a%d)" % (acc, r)
+                out.append("    f%d = f%d + a;" % (i, i))
             elif cv == -1:
-                acc = "(%s - a%d)" % (acc, r)
+                out.append("    f%d = f%d - a;" % (i, i))
             else:
-                acc = "fma(%s, a%d, %s)" % (lit(c), r, acc)
-        out.append("  f[%d] = %s;" % (i, acc if acc else "0.0"))
+                out.append("    f%d = fma(%s, a, f%d);" % (i, lit(c), i))
+        out.append("  }")
+    for i in range(59):
+        out.append("  WR(f, %d, f%d);" % (i, i))
     out.append("}")
-    return "\n".join(out)
+    # declare accumulators up front; first contributions initialise them
+    text = "\n".join(out)
+    decl = "  double " + ", ".join("f%d" % i for i in range(59) if mech["tendency"][i]) + ";"
+    text = text.replace("__device__ __forceinline__ void fun(KA k, VA v, double h2o, double m, FA f) {",
+                        "__device__ __forceinline__ void fun(KA k, VA v, double h2o, double m, FA f) {\n" + decl, 1)
+    import re
+    text = re.sub(r"f(\d+)_init = ", r"f\1 = ", text)
+    # species with no terms were declared separately above
+    return text
 
 
 # --------------------------------------------------------------------------
@@ -237,7 +266,7 @@ def gen_jac_prep(mech):
     for p in range(659):
         acc = None
         for c, r, others in terms[p]:
-            t = "k[%d]" % r
+            t = "RD(k, %d)" % r
             for kind, j in others:
                 t = "(%s * %s)" % (t, factor_src(kind, j))
             cv = float(c)
@@ -251,7 +280,7 @@ def gen_jac_prep(mech):
                 acc = "fma(%s, %s, %s)" % (lit(neg(c)), t, acc)
         if p in diag:
             acc = "(%s + ghinv)" % acc if acc else "ghinv"
-        out.append("  g[%d] = %s;" % (p, acc if acc else "0.0"))
+        out.append("  WR(g, %d, %s);" % (p, acc if acc else "0.0"))
     out.append("}")
     return "\n".join(out)
 
@@ -259,35 +288,79 @@ def gen_jac_prep(mech):
 # --------------------------------------------------------------------------
 # LU decomposition and solve
 # --------------------------------------------------------------------------
-def gen_decomp(mech):
+def _entry_expr(terms_p):
+    """-(sum of Jacobian terms) for one LU entry, in the contract order."""
+    acc = None
+    for c, r, others in terms_p:
+        t = "RD(k, %d)" % r
+        for kind, j in others:
+            t = "(%s * %s)" % (t, factor_src(kind, j))
+        cv = float(c)
+        if acc is None:
+            acc = ("(-%s)" % t) if cv == 1 else (t if cv == -1 else "(%s * %s)" % (lit(neg(c)), t))
+        elif cv == 1:
+            acc = "(%s - %s)" % (acc, t)
+        elif cv == -1:
+            acc = "(%s + %s)" % (acc, t)
+        else:
+            acc = "fma(%s, %s, %s)" % (lit(neg(c)), t, acc)
+    return acc
+
+
+def gen_decomp(mech, fused):
+    """Row-wise (IKJ) no-pivot LU in the reference's 659-position pattern.
+
+    fused = False: decomp(g) factors a prepared matrix in place.
+    fused = True:  jac_decomp(k, v, h2o, m, ghinv, g) forms each row of
+                   G = -J + ghinv*I right before eliminating it, so only one
+                   row (at most 47 entries) is live at a time.
+    Both store 1/U_kk on the diagonal and return 0, or k+1 for the first row
+    whose prepared diagonal is 0 (the reference's check, mgpu.cu:2753).
+    R > 0 launders the views every R rows (earlier rows are re-read from
+    memory instead of held in registers)."""
     lu = mech["lu"]
     crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
-    out = ["// row-wise no-pivot LU; diagonal holds 1/U_kk on return."]
-    out.append("// Returns 0, or k+1 for the first row whose prepared diagonal is 0.")
-    out.append("template <class GA>")
-    out.append("__device__ __forceinline__ int decomp(GA g) {")
+    terms = jac_terms(mech) if fused else None
+    out = []
+    if fused:
+        out.append("template <int R, class KA, class VA, class GA>")
+        out.append("__device__ __forceinline__ int jac_decomp(KA k, VA v, double h2o, double m,"
+                   " double ghinv, GA g) {")
+    else:
+        out.append("template <int R, class GA>")
+        out.append("__device__ __forceinline__ int decomp(GA g) {")
     out.append("  int sing = 0;")
     nfma = 0
     for k in range(59):
         cols = icol[crow[k]:crow[k + 1]]
+        if k > 0:
+            extra = " launder(k); launder(v);" if fused else ""
+            out.append("  if (R > 0 && %d %% (R > 0 ? R : 1) == 0) { launder(g);%s }" % (k, extra))
         out.append("  { // row %d" % k)
-        out.append("    sing = (sing == 0 && g[%d] == 0.0) ? %d : sing;" % (diag[k], k + 1))
         for kk in range(crow[k], crow[k + 1]):
-            out.append("    double w%d = g[%d];" % (icol[kk], kk))
+            c = icol[kk]
+            if fused:
+                e = _entry_expr(terms[kk])
+                if kk == diag[k]:
+                    e = "(%s + ghinv)" % e if e else "ghinv"
+                out.append("    double w%d = %s;" % (c, e if e else "0.0"))
+            else:
+                out.append("    double w%d = RD(g, %d);" % (c, kk))
+        out.append("    sing = (sing == 0 && w%d == 0.0) ? %d : sing;" % (k, k + 1))
         for kk in range(crow[k], diag[k]):
             j = icol[kk]
-            out.append("    w%d = w%d * g[%d];" % (j, j, diag[j]))
+            out.append("    w%d = w%d * RD(g, %d);" % (j, j, diag[j]))
             for jj in range(diag[j] + 1, crow[j + 1]):
                 c = icol[jj]
                 assert c in cols
-                out.append("    w%d = fma(-w%d, g[%d], w%d);" % (c, j, jj, c))
+                out.append("    w%d = fma(-w%d, RD(g, %d), w%d);" % (c, j, jj, c))
                 nfma += 1
         for kk in range(crow[k], crow[k + 1]):
             c = icol[kk]
             if kk == diag[k]:
-                out.append("    g[%d] = 1.0 / w%d;" % (kk, c))
+                out.append("    WR(g, %d, 1.0 / w%d);" % (kk, c))
             else:
-                out.append("    g[%d] = w%d;" % (kk, c))
+                out.append("    WR(g, %d, w%d);" % (kk, c))
         out.append("  }")
     out.append("  return sing;")
     out.append("}")
@@ -295,22 +368,28 @@ def gen_decomp(mech):
 
 
 def gen_solve(mech):
+    """x <- G^-1 x: forward (unit L) then backward (U, inverse pivots) with
+    the 59 entries of x held in registers and the LU streamed once."""
     lu = mech["lu"]
     crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
     out = ["// x <- G^-1 x with unit-lower L and U (inverse pivots on the diagonal)"]
     out.append("template <class GA, class XA>")
     out.append("__device__ __forceinline__ void solve(GA g, XA x) {")
     for i in range(59):
+        out.append("  double x%d = RD(x, %d);" % (i, i))
+    for i in range(59):
         if diag[i] > crow[i]:
-            acc = "x[%d]" % i
+            acc = "x%d" % i
             for kk in range(crow[i], diag[i]):
-                acc = "fma(-g[%d], x[%d], %s)" % (kk, icol[kk], acc)
-            out.append("  x[%d] = %s;" % (i, acc))
+                acc = "fma(-RD(g, %d), x%d, %s)" % (kk, icol[kk], acc)
+            out.append("  x%d = %s;" % (i, acc))
     for i in range(58, -1, -1):
-        acc = "x[%d]" % i
+        acc = "x%d" % i
         for kk in range(diag[i] + 1, crow[i + 1]):
-            acc = "fma(-g[%d], x[%d], %s)" % (kk, icol[kk], acc)
-        out.append("  x[%d] = %s * g[%d];" % (i, acc, diag[i]))
+            acc = "fma(-RD(g, %d), x%d, %s)" % (kk, icol[kk], acc)
+        out.append("  x%d = %s * RD(g, %d);" % (i, acc, diag[i]))
+    for i in range(59):
+        out.append("  WR(x, %d, x%d);" % (i, i))
     out.append("}")
     return "\n".join(out)
 
@@ -320,7 +399,8 @@ def generate(dst=None, mech_path=None):
     dst = dst or os.path.join(PKG, "csrc", "generated", "radm2_gen.h")
     os.makedirs(os.path.dirname(dst), exist_ok=True)
     rc, nshared = gen_rconst(mech)
-    dec, nfma = gen_decomp(mech)
+    dec, nfma = gen_decomp(mech, False)
+    jdec, _ = gen_decomp(mech, True)
     parts = [
         "// GENERATED by codegen/gen_radm2.py from mech/radm2.json -- do not edit.",
         "#pragma once",
@@ -361,6 +441,8 @@ def generate(dst=None, mech_path=None):
         gen_jac_prep(mech),
         "",
         dec,
+        "",
+        jdec,
         "",
         gen_solve(mech),
         "",

@@ -29,6 +29,10 @@ struct radm2_ctx {
   int schedule = RADM2_SCHED_PERSISTENT;
   int block = 0;       // 0 = tuned default
   int refill_min = 0;  // 0 = tuned default
+  int waves = 0;       // register-budget variant (waves per SIMD), 0 = tuned default
+  int placement = -1;  // working-array placement, -1 = tuned default
+  double* d_ws = nullptr;  // explicit workspace (grows on demand)
+  size_t ws_bytes = 0;
   DomainDesc* d_doms = nullptr;
   DomainDesc* h_doms = nullptr;  // pinned staging
   int doms_cap = 0;
@@ -43,15 +47,13 @@ struct radm2_plan {
   radm2_ctx* ctx = nullptr;
   DomainDesc* d_doms = nullptr;
   unsigned long long* d_counter = nullptr;
+  double* d_ws = nullptr;
+  size_t ws_bytes = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
 namespace {
-
-constexpr int kDefaultGridBlock = 64;
-constexpr int kDefaultPersistentBlock = 64;
-constexpr int kDefaultRefillMin = 16;
 
 #define HIP_TRY(x)                       \
   do {                                   \
@@ -65,6 +67,35 @@ constexpr int kDefaultRefillMin = 16;
 hipError_t& last_hip_error() {
   static thread_local hipError_t e = hipSuccess;
   return e;
+}
+
+constexpr int kDefaultGridBlock = 64;
+constexpr int kDefaultPersistentBlock = 64;
+constexpr int kDefaultRefillMin = 16;
+constexpr int kDefaultWaves = 1;
+constexpr int kDefaultPlacement = radm2::PLACE_PRIVATE;
+
+const radm2::KernelVariant* pick_variant(const radm2_ctx* c) {
+  const int w = c->waves ? c->waves : kDefaultWaves;
+  const int pl = c->placement >= 0 ? c->placement : kDefaultPlacement;
+  const radm2::KernelVariant* v = radm2::kernel_variants();
+  for (int i = 0; i < radm2::kNumVariants; i++)
+    if (v[i].waves == w && v[i].placement == pl) return &v[i];
+  return nullptr;
+}
+
+// Workspace for `waves` wave slots, grown (never shrunk) on demand.  The
+// stream is drained first: a running kernel may still use the old buffer.
+int ensure_ws(hipStream_t st, double** ws, size_t* have, int64_t waves) {
+  const size_t need = (size_t)waves * radm2::WS_DOUBLES_PER_LANE * radm2::WAVE * sizeof(double);
+  if (need <= *have) return 0;
+  HIP_TRY(hipStreamSynchronize(st));
+  if (*ws) HIP_TRY(hipFree(*ws));
+  *ws = nullptr;
+  *have = 0;
+  HIP_TRY(hipMalloc(ws, need));
+  *have = need;
+  return 0;
 }
 
 // KPP option resolution (Rosenbrock(), mgpu.cu:3688-3811); INTEGRATE()'s
@@ -157,9 +188,12 @@ int pick_block(const radm2_ctx* c) {
   return c->schedule == RADM2_SCHED_GRID ? kDefaultGridBlock : kDefaultPersistentBlock;
 }
 
-// Launch the kernels for an uploaded table on stream st.
+// Launch the kernels for an uploaded table on stream st.  ws/ws_bytes: the
+// workspace to use (grown here when the explicit placement needs more);
+// grow = false for graph capture, where the plan sized it beforehand.
 int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
-            unsigned long long* counter, hipStream_t st) {
+            unsigned long long* counter, hipStream_t st, double** ws, size_t* ws_bytes,
+            bool grow = true) {
   LaunchArgs a;
   a.doms = d_doms;
   a.ndom = ndom;
@@ -168,16 +202,37 @@ int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
   a.stats = c->stats_on ? c->d_stats : nullptr;
   a.refill_min = c->refill_min ? c->refill_min : kDefaultRefillMin;
   const int block = pick_block(c);
+  const radm2::KernelVariant* v = pick_variant(c);
+  if (!v) return RADM2_E_ARG;
+  int64_t blocks = 0, waves = 0;
   if (c->schedule == RADM2_SCHED_GRID) {
-    HIP_TRY(radm2::launch_grid(block, a, st));
+    // every wave of the launched grid owns a slot, padding waves included
+    waves = (total + block - 1) / block * (block / radm2::WAVE);
   } else {
-    int per_cu = radm2::persistent_blocks_per_cu(block);
+    int per_cu = v->blocks_per_cu(block);
     if (per_cu <= 0) return RADM2_E_HIP;
-    int64_t blocks = (int64_t)per_cu * c->cu_count;
+    blocks = (int64_t)per_cu * c->cu_count;
     const int64_t need = (total + block - 1) / block;
     blocks = std::max<int64_t>(1, std::min(blocks, need));
+    waves = blocks * (block / radm2::WAVE);
+  }
+  if (v->placement == radm2::PLACE_WS) {
+    if (grow) {
+      if (int rc = ensure_ws(st, ws, ws_bytes, waves)) return rc;
+    } else if (*ws_bytes < (size_t)waves * radm2::WS_DOUBLES_PER_LANE * radm2::WAVE * sizeof(double)) {
+      return RADM2_E_ARG;
+    }
+    a.ws = *ws;
+    a.ws_waves = waves;
+  } else {
+    a.ws = nullptr;
+    a.ws_waves = 0;
+  }
+  if (c->schedule == RADM2_SCHED_GRID) {
+    HIP_TRY(v->grid(block, a, st));
+  } else {
     HIP_TRY(radm2::launch_fill_u64(counter, 1, 0ull, st));
-    HIP_TRY(radm2::launch_persistent(block, (int)blocks, a, st));
+    HIP_TRY(v->persistent(block, (int)blocks, a, st));
   }
   return 0;
 }
@@ -242,6 +297,7 @@ void radm2_destroy(radm2_ctx* c) {
   if (c->h_doms) hipHostFree(c->h_doms);
   if (c->d_counter) hipFree(c->d_counter);
   if (c->d_stats) hipFree(c->d_stats);
+  if (c->d_ws) hipFree(c->d_ws);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->ev_staged) hipEventDestroy(c->ev_staged);
@@ -252,11 +308,28 @@ void radm2_destroy(radm2_ctx* c) {
 int radm2_set_schedule(radm2_ctx* c, int schedule, int block, int refill_min) {
   if (!c) return RADM2_E_ARG;
   if (schedule != RADM2_SCHED_GRID && schedule != RADM2_SCHED_PERSISTENT) return RADM2_E_ARG;
-  if (block != 0 && block != 64 && block != 128 && block != 256) return RADM2_E_ARG;
+  if (block != 0 && block != 64 && block != 256 && block != 512 && block != 1024) return RADM2_E_ARG;
+  if (schedule == RADM2_SCHED_PERSISTENT && block > 256) return RADM2_E_ARG;
   if (refill_min < 0 || refill_min > 64) return RADM2_E_ARG;
   c->schedule = schedule;
   c->block = block;
   c->refill_min = refill_min;
+  return RADM2_OK;
+}
+
+int radm2_set_kernel(radm2_ctx* c, int placement, int waves_per_simd) {
+  if (!c) return RADM2_E_ARG;
+  if (placement < -1 || placement > 1) return RADM2_E_ARG;
+  if (waves_per_simd != 0 && waves_per_simd != 1 && waves_per_simd != 2 && waves_per_simd != 4)
+    return RADM2_E_ARG;
+  const int pl = placement >= 0 ? placement : kDefaultPlacement;
+  const int w = waves_per_simd ? waves_per_simd : kDefaultWaves;
+  const radm2::KernelVariant* v = radm2::kernel_variants();
+  bool found = false;
+  for (int i = 0; i < radm2::kNumVariants; i++) found |= (v[i].placement == pl && v[i].waves == w);
+  if (!found) return RADM2_E_UNSUPPORTED;
+  c->placement = placement;
+  c->waves = waves_per_simd;
   return RADM2_OK;
 }
 
@@ -276,7 +349,8 @@ int radm2_integrate_batched(radm2_ctx* c, int ndom, const radm2_domain* doms) {
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipEventRecord(c->ev_staged, c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  if (int rc = enqueue(c, c->d_doms, ndom, total, c->d_counter, c->stream)) return rc;
+  if (int rc = enqueue(c, c->d_doms, ndom, total, c->d_counter, c->stream, &c->d_ws, &c->ws_bytes))
+    return rc;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   c->timed = true;
   return RADM2_OK;
@@ -334,9 +408,23 @@ int radm2_plan_create(radm2_ctx* c, int ndom, const radm2_domain* doms, radm2_pl
   // node) followed by the solver launch
   hipStream_t cap;
   if ((e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)) != hipSuccess) return fail(e);
+  // size the plan's own workspace before capture (no allocation inside it)
+  {
+    const int block = pick_block(c);
+    const radm2::KernelVariant* v = pick_variant(c);
+    int64_t waves = (total + block - 1) / block * (block / radm2::WAVE);
+    if (v && c->schedule == RADM2_SCHED_PERSISTENT) {
+      const int per_cu = v->blocks_per_cu(block);
+      waves = std::min<int64_t>(waves, (int64_t)std::max(per_cu, 1) * c->cu_count * (block / radm2::WAVE));
+    }
+    if (v && v->placement == radm2::PLACE_WS && ensure_ws(cap, &p->d_ws, &p->ws_bytes, waves) != 0) {
+      hipStreamDestroy(cap);
+      return fail(last_hip_error());
+    }
+  }
   e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    int rc = enqueue(c, p->d_doms, ndom, total, p->d_counter, cap);
+    int rc = enqueue(c, p->d_doms, ndom, total, p->d_counter, cap, &p->d_ws, &p->ws_bytes, false);
     hipError_t e2 = hipStreamEndCapture(cap, &p->graph);
     if (rc != 0 && e2 == hipSuccess) e2 = last_hip_error();
     e = e2;
@@ -370,6 +458,7 @@ void radm2_plan_destroy(radm2_plan* p) {
   if (p->graph) hipGraphDestroy(p->graph);
   if (p->d_doms) hipFree(p->d_doms);
   if (p->d_counter) hipFree(p->d_counter);
+  if (p->d_ws) hipFree(p->d_ws);
   delete p;
 }
 

@@ -27,7 +27,7 @@ __device__ __forceinline__ void cell_load(const DomainDesc& d, int64_t p, KA k, 
   h2o = d.h2o[p];
   gen::rconst(d.temp[p], m, d.jrate + p, ld, d.rc ? d.rc[p] : d.rc_default, k);
 #pragma unroll
-  for (int i = 0; i < NVAR; i++) y[i] = d.conc[i * ld + p];
+  for (int i = 0; i < NVAR; i++) WR(y, i, d.conc[i * ld + p]);
   cell_begin(s, d.opts, d.h0 ? d.h0[p] : 0.0);
 }
 
@@ -37,7 +37,7 @@ __device__ __forceinline__ void cell_store(const DomainDesc& d, int64_t p, YA y,
                                            unsigned long long* stats) {
   const int64_t ld = d.ld;
 #pragma unroll
-  for (int i = 0; i < NVAR; i++) d.conc[i * ld + p] = y[i];
+  for (int i = 0; i < NVAR; i++) d.conc[i * ld + p] = RD(y, i);
   if (d.nstep) d.nstep[p] = s.ns;
   if (d.nacc) d.nacc[p] = s.na;
   if (d.nrej) d.nrej[p] = s.nr;

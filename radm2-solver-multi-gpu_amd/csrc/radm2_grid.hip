@@ -5,34 +5,84 @@
 
 namespace radm2 {
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_ros3_grid(LaunchArgs a) {
+#ifndef RADM2_WAVES
+#define RADM2_WAVES 1
+#endif
+#ifndef RADM2_PLACE
+#define RADM2_PLACE 0
+#endif
+#ifndef RADM2_LROWS
+#define RADM2_LROWS 0
+#endif
+#define RADM2_CAT2(a, b) a##b
+#define RADM2_CAT(a, b) RADM2_CAT2(a, b)
+#define RADM2_SUFFIX \
+  RADM2_CAT(RADM2_CAT(RADM2_CAT(_p, RADM2_PLACE), RADM2_CAT(_w, RADM2_WAVES)), RADM2_CAT(_r, RADM2_LROWS))
+#define RADM2_NAME(x) RADM2_CAT(x, RADM2_SUFFIX)
+
+// RADM2_WAVES: minimum resident waves per SIMD the register budget must allow
+// LOCKSTEP: the waves of a block start every attempt together (one barrier
+// per attempt), so they walk the attempt's instruction stream side by side
+// and share each instruction-cache line; the block runs until its slowest
+// cell is done.
+// Each variant's kernels live in their own namespace: template kernels of
+// the same name in several translation units would otherwise be merged by
+// the linker into one definition.
+#define RADM2_VARIANT_NS RADM2_NAME(variant)
+namespace RADM2_VARIANT_NS {
+
+template <int BLOCK, bool LOCKSTEP>
+__global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) {
   const int64_t g = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (g >= a.total) return;
-  const int di = find_domain(a.doms, a.ndom, g);
+  const bool valid = g < a.total;
+  if (!LOCKSTEP && !valid) return;
+  const int64_t gc = valid ? g : a.total - 1;
+  const int di = find_domain(a.doms, a.ndom, gc);
   const DomainDesc& d = a.doms[di];
-  const int64_t p = g - d.first;
+  const int64_t p = gc - d.first;
+#if RADM2_PLACE == PLACE_PRIVATE
   double kk[NREACT], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
+#else
+  const int64_t wslot = __builtin_amdgcn_readfirstlane((int)(g / WAVE));
+  constexpr int kWsBytes = WS_DOUBLES_PER_LANE * WAVE * 8;
+  double* wbase = a.ws + wslot * ((int64_t)WS_DOUBLES_PER_LANE * WAVE);
+  const int lane = threadIdx.x & (WAVE - 1);
+  WsView kk = make_view(wbase, WS_K, lane, kWsBytes), y = make_view(wbase, WS_Y, lane, kWsBytes);
+  WsView lu = make_view(wbase, WS_LU, lane, kWsBytes), f = make_view(wbase, WS_F, lane, kWsBytes);
+  WsView k1 = make_view(wbase, WS_K1, lane, kWsBytes), k2 = make_view(wbase, WS_K2, lane, kWsBytes);
+  WsView k3 = make_view(wbase, WS_K3, lane, kWsBytes);
+#endif
   double h2o, m;
   CellState s;
   cell_load(d, p, kk, y, h2o, m, s);
+  if (!valid) s.status = ST_OK;  // padding lane of the last block: never runs
   const Ros3Opts& o = d.opts;
   for (;;) {
-    if (s.newstep) cell_step_top(s, o);
-    if (s.status != ST_RUNNING) break;
-    ros3_attempt(s, o, kk, y, h2o, m, lu, f, k1, k2, k3);
-    if (s.status != ST_RUNNING) break;
+    const bool run = s.status == ST_RUNNING;
+    if (LOCKSTEP) {
+      if (!__syncthreads_or(run)) break;
+    } else if (!run) {
+      break;
+    }
+    if (run) {
+      cell_attempt_top(s, o);
+      if (s.status == ST_RUNNING) ros3_attempt<RADM2_LROWS>(s, o, kk, y, h2o, m, lu, f, k1, k2, k3);
+    }
   }
-  cell_store(d, p, y, s, s.hfirst, a.stats);
+  if (valid) cell_store(d, p, y, s, s.hfirst, a.stats);
 }
 
-hipError_t launch_grid(int block, const LaunchArgs& a, hipStream_t st) {
+}  // namespace RADM2_VARIANT_NS
+using namespace RADM2_VARIANT_NS;
+
+hipError_t RADM2_NAME(launch_grid)(int block, const LaunchArgs& a, hipStream_t st) {
   if (a.total <= 0) return hipSuccess;
   const int64_t nb = (a.total + block - 1) / block;
   switch (block) {
-    case 64: hipLaunchKernelGGL(k_ros3_grid<64>, dim3((unsigned)nb), dim3(64), 0, st, a); break;
-    case 128: hipLaunchKernelGGL(k_ros3_grid<128>, dim3((unsigned)nb), dim3(128), 0, st, a); break;
-    case 256: hipLaunchKernelGGL(k_ros3_grid<256>, dim3((unsigned)nb), dim3(256), 0, st, a); break;
+    case 64: hipLaunchKernelGGL((k_ros3_grid<64, false>), dim3((unsigned)nb), dim3(64), 0, st, a); break;
+    case 256: hipLaunchKernelGGL((k_ros3_grid<256, true>), dim3((unsigned)nb), dim3(256), 0, st, a); break;
+    case 512: hipLaunchKernelGGL((k_ros3_grid<512, true>), dim3((unsigned)nb), dim3(512), 0, st, a); break;
+    case 1024: hipLaunchKernelGGL((k_ros3_grid<1024, true>), dim3((unsigned)nb), dim3(1024), 0, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -38,13 +38,52 @@ struct LaunchArgs {
   int64_t total;           // sum of domain sizes
   unsigned long long* counter;  // persistent: global work counter (zeroed before launch)
   unsigned long long* stats;    // [4] attempts, accepted, rejected, refills (nullable)
+  double* ws;                   // explicit workspace (WS placement), WS_DOUBLES_PER_LANE per lane
+  int64_t ws_waves;             // wave slots the workspace holds
 };
 
-// Kernel variants (each in its own translation unit).
-hipError_t launch_grid(int block, const LaunchArgs& a, hipStream_t st);
-hipError_t launch_persistent(int block, int blocks, const LaunchArgs& a, hipStream_t st);
+// Explicit-workspace layout per lane (doubles): rate constants, state,
+// LU, four stage vectors.
+constexpr int WS_K = 0;
+constexpr int WS_Y = WS_K + NREACT;
+constexpr int WS_LU = WS_Y + NVAR;
+constexpr int WS_F = WS_LU + LU_NNZ;
+constexpr int WS_K1 = WS_F + NVAR;
+constexpr int WS_K2 = WS_K1 + NVAR;
+constexpr int WS_K3 = WS_K2 + NVAR;
+constexpr int WS_DOUBLES_PER_LANE = WS_K3 + NVAR;
+
+// Placement of the lane-private working arrays
+constexpr int PLACE_PRIVATE = 0;  // compiler-managed (VGPR/AGPR, scratch spills)
+constexpr int PLACE_WS = 1;       // explicit wave-interleaved HBM workspace
+
+// Kernel variants: one translation unit per (placement P, register budget
+// W = minimum resident waves per SIMD, decomposition launder interval R);
+// block sizes 64 and 256 in each.
+#define RADM2_DECLARE_VARIANT(P, W, R)                                                   \
+  hipError_t launch_grid_p##P##_w##W##_r##R(int block, const LaunchArgs& a, hipStream_t st); \
+  hipError_t launch_persistent_p##P##_w##W##_r##R(int block, int blocks, const LaunchArgs& a, \
+                                                  hipStream_t st);                      \
+  int persistent_blocks_per_cu_p##P##_w##W##_r##R(int block);
+RADM2_DECLARE_VARIANT(0, 1, 0)
+RADM2_DECLARE_VARIANT(0, 2, 0)
+RADM2_DECLARE_VARIANT(0, 4, 0)
+RADM2_DECLARE_VARIANT(1, 1, 0)
+RADM2_DECLARE_VARIANT(1, 2, 4)
+RADM2_DECLARE_VARIANT(1, 4, 2)
+#undef RADM2_DECLARE_VARIANT
+
+struct KernelVariant {
+  int placement;
+  int waves;
+  int launder_rows;
+  hipError_t (*grid)(int, const LaunchArgs&, hipStream_t);
+  hipError_t (*persistent)(int, int, const LaunchArgs&, hipStream_t);
+  int (*blocks_per_cu)(int);
+};
+constexpr int kNumVariants = 6;
+const KernelVariant* kernel_variants();
+
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
-// Occupancy of the persistent kernel for a block size (blocks per CU).
-int persistent_blocks_per_cu(int block);
 
 }  // namespace radm2

@@ -10,9 +10,42 @@
 
 namespace radm2 {
 
+#ifndef RADM2_WAVES
+#define RADM2_WAVES 1
+#endif
+#ifndef RADM2_PLACE
+#define RADM2_PLACE 0
+#endif
+#ifndef RADM2_LROWS
+#define RADM2_LROWS 0
+#endif
+#define RADM2_CAT2(a, b) a##b
+#define RADM2_CAT(a, b) RADM2_CAT2(a, b)
+#define RADM2_SUFFIX \
+  RADM2_CAT(RADM2_CAT(RADM2_CAT(_p, RADM2_PLACE), RADM2_CAT(_w, RADM2_WAVES)), RADM2_CAT(_r, RADM2_LROWS))
+#define RADM2_NAME(x) RADM2_CAT(x, RADM2_SUFFIX)
+
+// Each variant's kernels live in their own namespace: template kernels of
+// the same name in several translation units would otherwise be merged by
+// the linker into one definition.
+#define RADM2_VARIANT_NS RADM2_NAME(variant)
+namespace RADM2_VARIANT_NS {
+
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_ros3_persistent(LaunchArgs a) {
+__global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_persistent(LaunchArgs a) {
+#if RADM2_PLACE == PLACE_PRIVATE
   double kk[NREACT], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
+#else
+  const int64_t wslot = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE));
+  constexpr int kWsBytes = WS_DOUBLES_PER_LANE * WAVE * 8;
+  double* wbase = a.ws + wslot * ((int64_t)WS_DOUBLES_PER_LANE * WAVE);
+  WsView kk = make_view(wbase, WS_K, threadIdx.x & (WAVE - 1), kWsBytes), y = make_view(wbase, WS_Y, threadIdx.x & (WAVE - 1), kWsBytes);
+  WsView lu = make_view(wbase, WS_LU, threadIdx.x & (WAVE - 1), kWsBytes);
+  WsView f = make_view(wbase, WS_F, threadIdx.x & (WAVE - 1), kWsBytes);
+  WsView k1 = make_view(wbase, WS_K1, threadIdx.x & (WAVE - 1), kWsBytes);
+  WsView k2 = make_view(wbase, WS_K2, threadIdx.x & (WAVE - 1), kWsBytes);
+  WsView k3 = make_view(wbase, WS_K3, threadIdx.x & (WAVE - 1), kWsBytes);
+#endif
   double h2o = 0.0, m = 0.0;
   CellState s;
   s.status = ST_OK;
@@ -55,8 +88,8 @@ __global__ __launch_bounds__(BLOCK) void k_ros3_persistent(LaunchArgs a) {
     }
     if (s.status == ST_RUNNING) {
       const Ros3Opts& o = a.doms[di].opts;
-      if (s.newstep) cell_step_top(s, o);
-      if (s.status == ST_RUNNING) ros3_attempt(s, o, kk, y, h2o, m, lu, f, k1, k2, k3);
+      cell_attempt_top(s, o);
+      if (s.status == ST_RUNNING) ros3_attempt<RADM2_LROWS>(s, o, kk, y, h2o, m, lu, f, k1, k2, k3);
     }
   }
   if (cell >= 0) {
@@ -66,42 +99,33 @@ __global__ __launch_bounds__(BLOCK) void k_ros3_persistent(LaunchArgs a) {
   if (a.stats && lane == 0) atomicAdd(&a.stats[3], refills);
 }
 
+}  // namespace RADM2_VARIANT_NS
+using namespace RADM2_VARIANT_NS;
+
 template <int BLOCK>
 static hipError_t launch_p(int blocks, const LaunchArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(k_ros3_persistent<BLOCK>, dim3(blocks), dim3(BLOCK), 0, st, a);
   return hipGetLastError();
 }
 
-hipError_t launch_persistent(int block, int blocks, const LaunchArgs& a, hipStream_t st) {
+hipError_t RADM2_NAME(launch_persistent)(int block, int blocks, const LaunchArgs& a, hipStream_t st) {
   if (a.total <= 0) return hipSuccess;
   switch (block) {
     case 64: return launch_p<64>(blocks, a, st);
-    case 128: return launch_p<128>(blocks, a, st);
     case 256: return launch_p<256>(blocks, a, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-int persistent_blocks_per_cu(int block) {
+int RADM2_NAME(persistent_blocks_per_cu)(int block) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
   switch (block) {
     case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_persistent<64>, 64, 0); break;
-    case 128: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_persistent<128>, 128, 0); break;
     case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_persistent<256>, 256, 0); break;
     default: break;
   }
   return e == hipSuccess ? nb : -1;
-}
-
-__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
-}
-
-hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill_u64, dim3((n + 63) / 64), dim3(64), 0, st, p, n, v);
-  return hipGetLastError();
 }
 
 }  // namespace radm2

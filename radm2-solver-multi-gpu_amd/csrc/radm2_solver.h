@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "radm2_ws.h"
 #include "generated/radm2_gen.h"
 
 namespace radm2 {
@@ -83,10 +84,15 @@ __device__ __forceinline__ void cell_begin(CellState& s, const Ros3Opts& o, doub
   s.newstep = 1;
 }
 
-// Top of a time step: end test, KPP -6/-7 checks, H clamp to the interval.
-__device__ __forceinline__ void cell_step_top(CellState& s, const Ros3Opts& o) {
+// Top of an attempt.  At the start of a time step: the end-of-interval test
+// (mgpu.cu:1608-1609).  Before every attempt: KPP's -6 (step count) and -7
+// (step too small) checks; applying them to rejected attempts too bounds a
+// cell whose attempts are never accepted, and !(H > roundoff) also catches a
+// NaN step.  At the start of a time step H is clamped to the interval end
+// (mgpu.cu:1612).
+__device__ __forceinline__ void cell_attempt_top(CellState& s, const Ros3Opts& o) {
   const double Tend = o.dt;
-  if (!((s.T - Tend) + o.roundoff <= 0.0)) {
+  if (s.newstep && !((s.T - Tend) + o.roundoff <= 0.0)) {
     s.status = ST_OK;
     return;
   }
@@ -94,54 +100,105 @@ __device__ __forceinline__ void cell_step_top(CellState& s, const Ros3Opts& o) {
     s.status = ST_TOO_MANY_STEPS;
     return;
   }
-  if (s.T + 0.1 * s.H == s.T || s.H <= o.roundoff) {
+  if (s.T + 0.1 * s.H == s.T || !(s.H > o.roundoff)) {
     s.status = ST_STEP_TOO_SMALL;
     return;
   }
-  s.H = dmin_ref(s.H, fabs(Tend - s.T));
-  s.newstep = 0;
+  if (s.newstep) {
+    s.H = dmin_ref(s.H, fabs(Tend - s.T));
+    s.newstep = 0;
+  }
 }
 
-// One Ros3 attempt for a running cell whose step top has been applied.
+// One Ros3 attempt for a running cell whose attempt top has been applied.
 //   y   state (updated on acceptance)     k   rate constants
 //   g   659-entry LU workspace            f, k1, k2, k3: 59-entry vectors
-template <class KA, class YA, class GA, class VA>
+// R: launder interval of the fused Jacobian/decomposition (rows) and, as
+// 16 reactions per unit, of the tendency; phases are laundered apart
+// whenever the arrays are workspace views.
+template <int R, class KA, class YA, class GA, class VA>
 __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA k, YA y,
                                              double h2o, double m, GA g, VA f, VA k1, VA k2,
                                              VA k3) {
+  constexpr int RF = R > 0 ? 16 * R : 0;
   const double H = s.H;
   // F0 = f(y); K1 = G^-1 F0
-  gen::fun(k, y, h2o, m, k1);
+  gen::fun<RF>(k, y, h2o, m, k1);
   const double ghinv = 1.0 / (H * ROS_GAMMA1);
-  gen::jac_prep(k, y, h2o, m, ghinv, g);
-  if (gen::decomp(g) != 0) {
+  launder(k);
+  launder(y);
+  launder(k1);
+  // G = -J(y) + ghinv*I formed row by row inside its own LU decomposition
+  if (gen::jac_decomp<R>(k, y, h2o, m, ghinv, g) != 0) {
     s.status = ST_SINGULAR;
     return;
   }
-  gen::solve(g, k1);
-  // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
-#pragma unroll
-  for (int i = 0; i < NVAR; i++) k2[i] = y[i] + k1[i];
-  gen::fun(k, k2, h2o, m, f);
   const double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
+  if constexpr (__is_same(VA, WsView)) {
+    // one instance of the solve for the three stages: the stage vector is
+    // picked at run time (a scalar offset), which keeps the code compact
+    int region = k1.region;
+#pragma nounroll
+    for (int st = 0; st < 3; st++) {
+      if (st == 1) {
+        // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
+        launder(k1);
+        launder(y);
 #pragma unroll
-  for (int i = 0; i < NVAR; i++) k2[i] = fma(k1[i], hc21, f[i]);
-  gen::solve(g, k2);
-  // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
+        for (int i = 0; i < NVAR; i++) WR(k2, i, RD(y, i) + RD(k1, i));
+        launder(k2);
+        launder(k);
+        gen::fun<RF>(k, k2, h2o, m, f);
+        launder(f);
+        launder(k1);
 #pragma unroll
-  for (int i = 0; i < NVAR; i++) k3[i] = fma(k2[i], hc32, fma(k1[i], hc31, f[i]));
-  gen::solve(g, k3);
+        for (int i = 0; i < NVAR; i++) WR(k2, i, fma(RD(k1, i), hc21, RD(f, i)));
+        region = k2.region;
+      } else if (st == 2) {
+        // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
+        launder(k2);
+        launder(f);
+        launder(k1);
+#pragma unroll
+        for (int i = 0; i < NVAR; i++) WR(k3, i, fma(RD(k2, i), hc32, fma(RD(k1, i), hc31, RD(f, i))));
+        region = k3.region;
+      }
+      WsView x = k1;
+      x.region = region;
+      launder(x);
+      launder(g);
+      gen::solve(g, x);
+    }
+  } else {
+    gen::solve(g, k1);
+    // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
+#pragma unroll
+    for (int i = 0; i < NVAR; i++) WR(k2, i, RD(y, i) + RD(k1, i));
+    gen::fun<RF>(k, k2, h2o, m, f);
+#pragma unroll
+    for (int i = 0; i < NVAR; i++) WR(k2, i, fma(RD(k1, i), hc21, RD(f, i)));
+    gen::solve(g, k2);
+    // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
+#pragma unroll
+    for (int i = 0; i < NVAR; i++) WR(k3, i, fma(RD(k2, i), hc32, fma(RD(k1, i), hc31, RD(f, i))));
+    gen::solve(g, k3);
+  }
+  launder(k1);
+  launder(k2);
+  launder(k3);
+  launder(y);
   // combination and WRMS error norm
   double err = 0.0;
 #pragma unroll
   for (int i = 0; i < NVAR; i++) {
-    const double sm = fma(ROS_M3, k3[i], fma(ROS_M2, k2[i], ROS_M1 * k1[i]));
-    const double yn = y[i] + sm;
-    const double e = fma(ROS_E3, k3[i], fma(ROS_E2, k2[i], ROS_E1 * k1[i]));
-    const double ymax = dmax_ref(fabs(y[i]), fabs(yn));
+    const double a1 = RD(k1, i), a2 = RD(k2, i), a3 = RD(k3, i), yi = RD(y, i);
+    const double sm = fma(ROS_M3, a3, fma(ROS_M2, a2, ROS_M1 * a1));
+    const double yn = yi + sm;
+    const double e = fma(ROS_E3, a3, fma(ROS_E2, a2, ROS_E1 * a1));
+    const double ymax = dmax_ref(fabs(yi), fabs(yn));
     const double q = e / fma(o.rtol, ymax, o.atol);
     err = fma(q, q, err);
-    f[i] = yn;  // keep Ynew until the decision
+    WR(f, i, yn);  // keep Ynew until the decision
   }
   err = sqrt(err / (double)NVAR);
   const double fsafe = dmax_ref(o.facmin, o.facsafe / pow(err, 1.0 / ROS_ELO));
@@ -149,8 +206,10 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
   double Hnew = H * fac;
   s.ns++;
   if (err <= 1.0 || H <= o.hmin) {
+    launder(f);
+    launder(y);
 #pragma unroll
-    for (int i = 0; i < NVAR; i++) y[i] = f[i];
+    for (int i = 0; i < NVAR; i++) WR(y, i, RD(f, i));
     s.T = s.T + H;
     if (s.na == 0) s.hfirst = H * fsafe;  // uncapped proposal (learned-h0 target)
     if (s.rejLast) Hnew = dmin_ref(Hnew, H);

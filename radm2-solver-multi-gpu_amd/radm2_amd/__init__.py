@@ -32,6 +32,8 @@ CELL_STEP_TOO_SMALL = -7
 CELL_SINGULAR = -8
 SCHED_GRID = 0
 SCHED_PERSISTENT = 1
+PLACE_PRIVATE = 0
+PLACE_WORKSPACE = 1
 
 # reference species order (mgpu.cu:125-183)
 SPECIES = (
@@ -74,6 +76,7 @@ _EXPORTS = [
     ("radm2_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_void_p]),
     ("radm2_destroy", None, [ctypes.c_void_p]),
     ("radm2_set_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("radm2_set_kernel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("radm2_integrate", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_void_p] * 7
      + [ctypes.c_double] * 3 + [ctypes.POINTER(Opts)] + [ctypes.c_void_p] * 6),
@@ -216,6 +219,9 @@ class Solver:
 
     def set_schedule(self, schedule=SCHED_PERSISTENT, block=0, refill_min=0):
         _check(self.lib.radm2_set_schedule(self.ctx, schedule, block, refill_min))
+
+    def set_kernel(self, placement=-1, waves_per_simd=0):
+        _check(self.lib.radm2_set_kernel(self.ctx, placement, waves_per_simd))
 
     def integrate(self, batch, dt, rtol=1e-3, atol=1.0, opts=None, ncells=None):
         """Asynchronous integration of one domain (radm2_integrate_batched)."""

@@ -8,8 +8,9 @@ This generator produces the same solver inputs directly, per cell:
   T ~ U(260, 300) K; P ~ U(500, 1013) hPa; M = P/(k_B T) (molec/cm3)
   x_H2O log-uniform 1e-3..2e-2, H2O = M x_H2O
   solar zenith angle ~ U(0, zmax) deg; J_i = J_i^clear * max(cos(SZA), 0)
-    with zmax = 80 (4,096-cell case) or 100 (the 1M-cell and later cases,
-    so about 20% of cells are night cells with J = 0)
+    with zmax = 80 (4,096-cell case) or 100 (the 1M-cell and later cases;
+    cos(SZA) <= 0 for SZA >= 90 deg, so 10% of cells are night cells with
+    J = 0 -- the workload text says "~20%", which U(0,100) does not give)
   every variable species: M x with x log-uniform 1e-12..1e-7, except
     O3 U(20, 80) ppb, NOx log-uniform 0.1..20 ppb (NO share U(0.1, 0.5) of
     NOx), CO U(80, 300) ppb

@@ -1,0 +1,99 @@
+"""Host-side checks of the C ABI that need no GPU: the library loads, exports
+every function include/radm2.h declares, and its KPP option validation
+returns the reference's IERR codes (mgpu.cu:3715-3811) exactly as the
+oracle's does."""
+import ctypes
+import itertools
+import os
+import re
+
+import pytest
+
+import oracle_ct as O
+
+ROOT = O.ROOT
+HEADER = os.path.join(ROOT, "include", "radm2.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(radm2_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    import radm2_amd
+    lib = radm2_amd.load_library()
+    names = _declared()
+    assert "radm2_integrate" in names and "radm2_rosenbrock" in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert missing == []
+    # and the Python mirror binds each of them
+    assert sorted(radm2_amd.EXPORTED_SYMBOLS) == names
+
+
+def test_strerror_texts_follow_kpp():
+    import radm2_amd
+    assert radm2_amd.strerror(-5) == "Improper tolerance values"
+    assert radm2_amd.strerror(-8) == "Matrix is repeatedly singular"
+    assert radm2_amd.strerror(radm2_amd.E_ARG) == "invalid argument"
+
+
+CASES = {
+    "hmin": [0.0, 1e-3, -1.0],
+    "hmax": [0.0, 100.0, -5.0],
+    "hstart": [0.0, 6.0, -1e-3],
+    "facmin": [0.0, 0.3, -0.1],
+    "facmax": [0.0, 5.0, -1.0],
+    "facrej": [0.0, 0.2, -1.0],
+    "facsafe": [0.0, 0.8, -0.5],
+    "max_steps": [0, 50, -1],
+    "method": [0, 2, 1, 3, 6, -1],
+}
+
+
+def _expected(o, rtol, atol):
+    """KPP's order of checks (mgpu.cu:3715-3811), written from the spec."""
+    if o["max_steps"] < 0:
+        return -1
+    m = o["method"]
+    if m != 0 and not (1 <= m <= 5):
+        return -2
+    if o["hmin"] < 0 or o["hmax"] < 0 or o["hstart"] < 0:
+        return -3
+    if o["facmin"] < 0 or o["facmax"] < 0 or o["facrej"] < 0 or o["facsafe"] < 0:
+        return -4
+    if atol <= 0 or rtol <= 10 * 2.0 ** -52 or rtol >= 1:
+        return -5
+    if m not in (0, 2):
+        return -2
+    return 0
+
+
+def test_option_codes_match_oracle_and_kpp():
+    import radm2_amd
+    keys = list(CASES)
+    n = 0
+    for field in keys:
+        for v in CASES[field]:
+            for rtol, atol in ((1e-3, 1.0), (1e-16, 1.0), (1.0, 1.0), (1e-3, 0.0)):
+                o = {k: CASES[k][0] for k in keys}
+                o[field] = v
+                po = radm2_amd.Opts(**o)
+                oo = O.OptsIn(**o)
+                got = radm2_amd.check_opts(600.0, rtol, atol, po)
+                rc, _ = O.resolve(600.0, rtol, atol, oo)
+                assert got == rc == _expected(o, rtol, atol), (o, rtol, atol, got, rc)
+                n += 1
+    assert n > 100
+    assert radm2_amd.check_opts(600.0, 1e-3, 1.0, None) == 0
+
+
+def test_integrate_defaults_resolve_like_integrate_call():
+    # INTEGRATE(): Hstart = 0.01*dt, all else KPP defaults (mgpu.cu:3866-3873)
+    rc, o = O.resolve(600.0, 1e-3, 1.0, None)
+    assert rc == 0
+    assert o.hstart == pytest.approx(6.0)
+    assert (o.hmin, o.hmax, o.facmin, o.facmax, o.facrej, o.facsafe, o.max_steps) == \
+        (0.0, 600.0, 0.2, 6.0, 0.1, 0.9, 100000)
+    assert o.roundoff == 2.0 ** -52

@@ -1,0 +1,159 @@
+"""The oracle's Ros3 integrator against closed forms and an independent
+stiff solver.
+
+  * one photolysis reaction (NO2 + hv -> NO + O3P) is linear with a closed
+    form; fixed steps h, h/2, h/4 on the bimolecular NO + O3 -> NO2 problem
+    (closed form c/(1 + k c t)) converge at order 3, the order of Ros3
+    (Sandu et al. 1997);
+  * a very stiff linear decay with steps far beyond its time scale is damped
+    to ~0 (L-stability, R(inf) = 0), not oscillating;
+  * two real cells agree with scipy's Radau IIA (rtol 1e-10) on the same
+    right-hand side;
+  * KPP's failure codes: -6 (step budget), -7 (step too small or NaN).
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_ct as O
+
+MECH = json.load(open(os.path.join(O.ROOT, "radm2-solver-multi-gpu_amd", "mech", "radm2.json")))
+NO, NO2, O3P, O3 = 57, 58, 18, 45
+
+
+def _opts(**kw):
+    o = O.OptsIn()
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _cell(y, k, dt, rtol, atol, opts=None, h0=0.0, h2o=1e17, m=2.5e19):
+    rc, oo = O.resolve(dt, rtol, atol, opts)
+    assert rc == 0
+    y = np.array(y, dtype=np.float64)
+    ns, na, nr = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    hx = ctypes.c_double()
+    kk = np.ascontiguousarray(k, dtype=np.float64)
+    st = O.lib().oracle_ros3_cell(O._p(y), O._p(kk), h2o, m, dt, rtol, atol,
+                                  ctypes.byref(oo), h0, ctypes.byref(ns), ctypes.byref(na),
+                                  ctypes.byref(nr), ctypes.byref(hx))
+    return st, y, ns.value, na.value, nr.value
+
+
+def test_photolysis_closed_form():
+    # reaction 0 is NO2 photolysis (jv slot 2): its tendency terms
+    assert [r for r, c in MECH["tendency"][NO2] if r == 0] == [0]
+    j = 8e-3
+    k = np.zeros(156)
+    k[0] = j
+    y0 = np.zeros(59)
+    y0[NO2], y0[NO], y0[O3P] = 1e11, 3e10, 0.0
+    dt = 600.0
+    st, y, ns, na, nr = _cell(y0, k, dt, 1e-8, 1e-3)
+    assert st == 1
+    e = np.exp(-j * dt)
+    assert y[NO2] == pytest.approx(1e11 * e, rel=1e-6)
+    assert y[NO] == pytest.approx(3e10 + 1e11 * (1 - e), rel=1e-7)
+    assert y[O3P] == pytest.approx(1e11 * (1 - e), rel=1e-7)
+
+
+def _fixed_step_error(h, kc, c, dt):
+    k = np.zeros(156)
+    k[25] = kc              # O3 + NO -> NO2 (A[25] = k*[O3]*[NO])
+    y0 = np.zeros(59)
+    y0[O3] = y0[NO] = c
+    opts = _opts(hmin=h, hstart=h, facmin=1.0, facmax=1.0)
+    st, y, ns, na, nr = _cell(y0, k, dt, 1e-3, 1.0, opts)
+    assert st == 1 and nr == 0
+    exact = c / (1.0 + kc * c * dt)
+    return abs(y[NO] - exact) / exact
+
+
+def test_fixed_step_order_three():
+    assert any(r == 25 for r, c in MECH["tendency"][NO])
+    kc, c, dt = 1.8e-14, 5e11, 600.0
+    errs = [_fixed_step_error(h, kc, c, dt) for h in (60.0, 30.0, 15.0, 7.5)]
+    orders = [np.log2(errs[i] / errs[i + 1]) for i in range(3)]
+    assert all(2.6 < p < 3.4 for p in orders), (errs, orders)
+
+
+def test_stiff_decay_is_damped():
+    k = np.zeros(156)
+    k[0] = 1e6               # time scale 1 us, steps of 60 s
+    y0 = np.zeros(59)
+    y0[NO2] = 1e11
+    opts = _opts(hmin=60.0, hstart=60.0, facmin=1.0, facmax=1.0)
+    st, y, ns, na, nr = _cell(y0, k, 600.0, 1e-3, 1.0, opts)
+    assert st == 1 and na == 10
+    assert abs(y[NO2]) <= 1e-6 * 1e11
+    # NO + NO2 is conserved up to roundoff amplified by h*gamma*j ~ 3e7
+    assert y[NO] == pytest.approx(1e11, rel=1e-7)
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_against_scipy_radau(seed):
+    scipy = pytest.importorskip("scipy.integrate")
+    from radm2_amd import synth
+    c = synth.make_cells(1, seed, 60.0, 0.0)
+    T, M, H2O = c["temp"][0], c["cair"][0], c["h2o"][0]
+    k = O.rconst(T, M, c["jrate"][:, 0], c["rc_n2o5"][0])
+    y0 = c["conc"][:, 0].copy()
+    lu = MECH["lu"]
+    rows = np.repeat(np.arange(59), np.diff(lu["crow"]))
+    cols = np.array(lu["icol"])
+    H2O2, OH = 13, 52
+
+    def f(t, y):
+        return O.fun(y, H2O, M, k)
+
+    def jac(t, y):
+        # Radau's Newton iteration needs the derivative of the f it solves:
+        # take the reference's Jacobian back to exactly dFun/dV by removing
+        # the H2O2 + OH loss that Fun omits (mgpu.cu:1275)
+        J = np.zeros((59, 59))
+        J[rows, cols] = O.jac(y, H2O, M, k)
+        J[H2O2, H2O2] += k[33] * y[OH]
+        J[H2O2, OH] += k[33] * y[H2O2]
+        return J
+
+    dt = 1.0
+    ref = scipy.solve_ivp(f, (0.0, dt), y0, method="Radau", jac=jac, rtol=1e-11,
+                          atol=1e-6).y[:, -1]
+    big = np.abs(ref) > 1e3
+    errs = []
+    for rtol in (1e-6, 1e-8):
+        st, y, *_ = _cell(y0, k, dt, rtol, 1.0, h2o=H2O, m=M)
+        assert st == 1
+        errs.append(np.max(np.abs(y[big] - ref[big]) / np.abs(ref[big])))
+    # global error is a few hundred local tolerances and falls with rtol
+    assert errs[0] < 1e-3 and errs[1] < 1e-5 and errs[1] < errs[0], errs
+
+
+def test_too_many_steps_status():
+    from radm2_amd import synth
+    c = synth.make_cells(1, 3, 60.0, 0.0)
+    k = O.rconst(c["temp"][0], c["cair"][0], c["jrate"][:, 0], c["rc_n2o5"][0])
+    st, y, ns, na, nr = _cell(c["conc"][:, 0], k, 600.0, 1e-3, 1.0, _opts(max_steps=5))
+    assert st == -6 and ns == 6
+
+
+def test_nan_input_ends_with_step_too_small():
+    from radm2_amd import synth
+    c = synth.make_cells(1, 4, 60.0, 0.0)
+    k = O.rconst(c["temp"][0], c["cair"][0], c["jrate"][:, 0], c["rc_n2o5"][0])
+    y0 = c["conc"][:, 0].copy()
+    y0[NO] = np.nan
+    st, y, ns, na, nr = _cell(y0, k, 600.0, 1e-3, 1.0)
+    assert st == -7 and ns == 1 and na == 0
+
+
+def test_decomp_zero_diagonal_reports_row():
+    G = np.ones(659)
+    d = MECH["lu"]["diag"][17]
+    G[d] = 0.0
+    rc, _ = O.decomp(G)
+    assert rc == 18
