@@ -108,7 +108,7 @@ int radm2_set_schedule(radm2_ctx *ctx, int schedule, int block, int refill_min);
  * (compiler-managed registers, spills to scratch) or RADM2_PLACE_WORKSPACE
  * (an explicit wave-interleaved HBM workspace, streamed phase by phase);
  * -1 = tuned default -- and the register budget as the minimum resident
- * waves per SIMD (1, 2 or 4; 0 = tuned default).  Returns
+ * waves per SIMD (1, 2, 4 or 8; 0 = tuned default).  Returns
  * RADM2_E_UNSUPPORTED for a combination that is not compiled in.  No
  * reference counterpart. */
 #define RADM2_PLACE_PRIVATE 0

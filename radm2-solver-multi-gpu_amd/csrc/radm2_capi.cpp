@@ -71,8 +71,11 @@ hipError_t& last_hip_error() {
 
 constexpr int kDefaultGridBlock = 64;
 constexpr int kDefaultPersistentBlock = 64;
-constexpr int kDefaultRefillMin = 16;
-constexpr int kDefaultWaves = 1;
+// Defaults measured on MI355X at 1M cells (the autotuner may override):
+// persistent schedule, compiler-managed placement at >= 4 waves per SIMD,
+// refill as soon as 4 lanes of a wave are idle.
+constexpr int kDefaultRefillMin = 4;
+constexpr int kDefaultWaves = 4;
 constexpr int kDefaultPlacement = radm2::PLACE_PRIVATE;
 
 const radm2::KernelVariant* pick_variant(const radm2_ctx* c) {
@@ -320,7 +323,8 @@ int radm2_set_schedule(radm2_ctx* c, int schedule, int block, int refill_min) {
 int radm2_set_kernel(radm2_ctx* c, int placement, int waves_per_simd) {
   if (!c) return RADM2_E_ARG;
   if (placement < -1 || placement > 1) return RADM2_E_ARG;
-  if (waves_per_simd != 0 && waves_per_simd != 1 && waves_per_simd != 2 && waves_per_simd != 4)
+  if (waves_per_simd != 0 && waves_per_simd != 1 && waves_per_simd != 2 && waves_per_simd != 4 &&
+      waves_per_simd != 8)
     return RADM2_E_ARG;
   const int pl = placement >= 0 ? placement : kDefaultPlacement;
   const int w = waves_per_simd ? waves_per_simd : kDefaultWaves;

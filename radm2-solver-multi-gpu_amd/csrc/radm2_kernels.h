@@ -58,7 +58,7 @@ constexpr int PLACE_PRIVATE = 0;  // compiler-managed (VGPR/AGPR, scratch spills
 constexpr int PLACE_WS = 1;       // explicit wave-interleaved HBM workspace
 
 // Kernel variants: one translation unit per (placement P, register budget
-// W = minimum resident waves per SIMD, decomposition launder interval R);
+// W = minimum resident waves per SIMD (1, 2, 4, 8), decomposition launder interval R);
 // block sizes 64 and 256 in each.
 #define RADM2_DECLARE_VARIANT(P, W, R)                                                   \
   hipError_t launch_grid_p##P##_w##W##_r##R(int block, const LaunchArgs& a, hipStream_t st); \
@@ -68,6 +68,7 @@ constexpr int PLACE_WS = 1;       // explicit wave-interleaved HBM workspace
 RADM2_DECLARE_VARIANT(0, 1, 0)
 RADM2_DECLARE_VARIANT(0, 2, 0)
 RADM2_DECLARE_VARIANT(0, 4, 0)
+RADM2_DECLARE_VARIANT(0, 8, 0)
 RADM2_DECLARE_VARIANT(1, 1, 0)
 RADM2_DECLARE_VARIANT(1, 2, 4)
 RADM2_DECLARE_VARIANT(1, 4, 2)
@@ -81,7 +82,7 @@ struct KernelVariant {
   hipError_t (*persistent)(int, int, const LaunchArgs&, hipStream_t);
   int (*blocks_per_cu)(int);
 };
-constexpr int kNumVariants = 6;
+constexpr int kNumVariants = 7;
 const KernelVariant* kernel_variants();
 
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
