@@ -41,10 +41,12 @@ def parse():
     ap.add_argument("--rtol", type=float, default=1e-3)
     ap.add_argument("--atol", type=float, default=1.0)
     ap.add_argument("--paths", default="grid,persistent")
-    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--refill", type=int, default=0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device copy (calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -82,6 +84,16 @@ def main():
               "see DESIGN.md)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
+
+    if args.copy_probe:
+        # a plain copy of known size: 1 GiB read + 1 GiB written
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 8}))
+        return
 
     import radm2_amd
     from radm2_amd import synth, work
@@ -134,14 +146,28 @@ def main():
         "roofline": {"bound": "mfma", "compute": "FP64 VALU (no MFMA on this path); "
                      "MI355X FP64 vector peak = 78.6 TFLOP/s",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "flop_per_launch": flop, "kernel_ms": kms},
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_bytes(r),
+                     "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE per attempt, calibrated "
+                                       "with a 1 GiB copy (profiles/radm2_persistent_1m.md), "
+                                       "scaled to this launch's attempts",
+                     "flop_per_launch": flop, "kernel_ms": kms,
+                     "hbm_gbs": (traffic_bytes(r) or 0) / (kms * 1e-3) / 1e9},
     }
     if "grid" in results and "persistent" in results:
         out["persistent_vs_grid"] = (results["grid"]["wall_s"] / results["persistent"]["wall_s"])
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cells, args)
     print(json.dumps(out), flush=True)
+
+
+def traffic_bytes(r):
+    """HBM bytes of one launch: measured bytes per attempt times attempts."""
+    p = os.path.join(ROOT, "profiles", "radm2_persistent_1m_traffic.json")
+    if not os.path.exists(p):
+        return None
+    t = json.load(open(p))
+    per_attempt = (t["read_bytes"] + t["write_bytes"]) / t["attempts"]
+    return per_attempt * (r["nacc"] + r["nrej"])
 
 
 def cpu_baseline(cells, args):

@@ -12,13 +12,17 @@ import pytest
 import oracle_ct as O
 
 ROOT = O.ROOT
-HEADER = os.path.join(ROOT, "include", "radm2.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if h.endswith(".h")]
 
 
 def _declared():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(radm2_\w+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(radm2_\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -29,7 +33,8 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert missing == []
     # and the Python mirror binds each of them
-    assert sorted(radm2_amd.EXPORTED_SYMBOLS) == names
+    from radm2_amd import mlp
+    assert sorted(radm2_amd.EXPORTED_SYMBOLS + mlp.EXPORTED_SYMBOLS) == names
 
 
 def test_strerror_texts_follow_kpp():
