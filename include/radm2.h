@@ -83,7 +83,7 @@ typedef struct radm2_domain {
   const double *h2o;      /* [ld] water (molec/cm3) */
   const double *jrate;    /* [21][ld] photolysis rates (s^-1) */
   const double *rc_n2o5;  /* [ld] N2O5 hydrolysis rate, NULL = reference value at RH 0 */
-  const double *h0;       /* [ld] initial step per cell, NULL = options' Hstart */
+  const double *h0;       /* [ld] per-cell Hstart (entries <= 0 and NULL = the options') */
   int32_t *nstep, *nacc, *nrej, *status; /* [ld] per-cell outputs, each nullable */
   double *hexit;          /* [ld] step size at exit (nullable) */
   double *hfirst;         /* [ld] controller proposal after the first accepted step (nullable) */

@@ -68,7 +68,7 @@ int radm2_mlp_features(void *stream, int64_t n, int64_t ld, const double *conc,
                        const double *jrate, const float *mu, const float *sd, int F, uint16_t *X,
                        float *Xf);
 
-/* h0[p] = clamp(10^out[p,0], 1e-6, hmax) if sigmoid(out[p,1]) > threshold,
+/* h0[p] = clamp(10^out[p,0], 1e-12, hmax) if sigmoid(out[p,1]) > threshold,
  * else fallback (0 = the options' Hstart); used[p] (nullable) = 1 if taken. */
 int radm2_mlp_h0_select(void *stream, int64_t n, int N, const float *out, double hmax,
                         float threshold, double fallback, double *h0, int32_t *used);

@@ -249,7 +249,10 @@ int oracle_ros3_cell(double *y, const double *k, double h2o, double m, double dt
   double T = 0.0;
   double H = O_MIN(o->hstart, o->hmax);
   if (fabs(H) <= 10.0 * rnd) H = DELTA_MIN;
-  if (h0 > 0.0) H = O_MIN(O_MAX(h0, O_MAX(o->hmin, DELTA_MIN)), o->hmax);
+  if (h0 > 0.0) { /* per-cell user Hstart: mgpu.cu:3752, then 3475-3477 */
+    H = O_MIN(O_MIN(h0, dt), o->hmax);
+    if (fabs(H) <= 10.0 * rnd) H = DELTA_MIN;
+  }
   int rejLast = 0, rejMore = 0, status = 1;
   int32_t ns = 0, na = 0, nr = 0;
   while ((T - Tend) + rnd <= 0.0) {

@@ -227,7 +227,7 @@ __global__ void k_features(int64_t n, int64_t ld, const double* __restrict__ con
 }
 
 // Learned initial step from the network outputs: use 10^out[0] clamped to
-// [1e-6, hmax] when sigmoid(out[1]) > threshold, else `fallback` (0 selects
+// [1e-12, hmax] when sigmoid(out[1]) > threshold, else `fallback` (0 selects
 // the options' Hstart in the solver).
 __global__ void k_h0_select(int64_t n, int N, const float* __restrict__ out, double hmax,
                             float threshold, double fallback, double* __restrict__ h0,
@@ -237,7 +237,7 @@ __global__ void k_h0_select(int64_t n, int N, const float* __restrict__ out, dou
   const float conf = 1.0f / (1.0f + __expf(-out[p * N + 1]));
   const bool use = conf > threshold;
   double h = pow(10.0, (double)out[p * N]);
-  h = fmin(fmax(h, 1.0e-6), hmax);
+  h = fmin(fmax(h, 1.0e-12), hmax);
   h0[p] = use ? h : fallback;
   if (used) used[p] = use ? 1 : 0;
 }

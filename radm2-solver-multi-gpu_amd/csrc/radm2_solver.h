@@ -76,7 +76,12 @@ __device__ __forceinline__ void cell_begin(CellState& s, const Ros3Opts& o, doub
   s.hfirst = 0.0;
   double H = dmin_ref(o.hstart, o.hmax);
   if (fabs(H) <= 10.0 * o.roundoff) H = DELTA_MIN;
-  if (h0 > 0.0) H = dmin_ref(dmax_ref(h0, dmax_ref(o.hmin, DELTA_MIN)), o.hmax);
+  // a per-cell h0 is a user-selected Hstart for that cell: Hstart =
+  // min(h0, dt) (mgpu.cu:3752), H = min(Hstart, Hmax) (mgpu.cu:3475-3477)
+  if (h0 > 0.0) {
+    H = dmin_ref(dmin_ref(h0, o.dt), o.hmax);
+    if (fabs(H) <= 10.0 * o.roundoff) H = DELTA_MIN;
+  }
   s.H = H;
   s.ns = s.na = s.nr = 0;
   s.status = ST_RUNNING;

@@ -198,13 +198,38 @@ class CellBatch:
 
 
 class Solver:
-    """A context on one device (radm2_create / radm2_destroy)."""
+    """A context on one device (radm2_create / radm2_destroy).
+
+    By default the context runs on a torch stream of its own, ordered with
+    torch's current stream on every enqueue: the solver stream first waits
+    for torch's current stream (inputs written by torch operations are
+    complete), then torch's current stream waits for the solver stream
+    (torch operations on the outputs see the results).  Both waits are
+    device-side.  With an explicit ``stream`` (a hipStream_t handle) the
+    ordering is the caller's job, as in the C ABI.
+    """
 
     def __init__(self, device=0, stream=None):
+        import torch
         self.lib = load_library()
         self.ctx = ctypes.c_void_p()
+        self.device = torch.device("cuda", int(device))
+        self._ts = None
+        if stream is None:
+            self._ts = torch.cuda.Stream(device=self.device)
+            stream = self._ts.cuda_stream
         _check(self.lib.radm2_create(ctypes.byref(self.ctx), int(device),
-                                     ctypes.c_void_p(stream) if stream else None))
+                                     ctypes.c_void_p(stream)))
+
+    def _before(self):
+        if self._ts is not None:
+            import torch
+            self._ts.wait_stream(torch.cuda.current_stream(self.device))
+
+    def _after(self):
+        if self._ts is not None:
+            import torch
+            torch.cuda.current_stream(self.device).wait_stream(self._ts)
 
     def close(self):
         if self.ctx:
@@ -226,12 +251,17 @@ class Solver:
     def integrate(self, batch, dt, rtol=1e-3, atol=1.0, opts=None, ncells=None):
         """Asynchronous integration of one domain (radm2_integrate_batched)."""
         d = batch.domain(dt, rtol, atol, opts, ncells)
+        self._before()
         code = self.lib.radm2_integrate_batched(self.ctx, 1, ctypes.byref(d))
+        self._after()
         _check(code)
 
     def integrate_batched(self, domains):
         arr = (Domain * len(domains))(*domains)
-        _check(self.lib.radm2_integrate_batched(self.ctx, len(domains), arr))
+        self._before()
+        code = self.lib.radm2_integrate_batched(self.ctx, len(domains), arr)
+        self._after()
+        _check(code)
 
     def plan(self, domains):
         arr = (Domain * len(domains))(*domains)
@@ -262,7 +292,9 @@ class Solver:
         rt = (ctypes.c_double * len(RelTol))(*RelTol)
         rp = (ctypes.c_double * 20)(*RPAR)
         ip = (ctypes.c_int32 * 20)(*IPAR)
+        self._before()
         code = self.lib.radm2_rosenbrock(self.ctx, ctypes.byref(d), Tstart, Tend, at, rt, rp, ip)
+        self._after()
         RPAR[:] = list(rp)
         IPAR[:] = list(ip)
         return code
@@ -276,7 +308,10 @@ class Plan:
         self.handle = handle
 
     def launch(self):
-        _check(self.solver.lib.radm2_plan_launch(self.handle))
+        self.solver._before()
+        code = self.solver.lib.radm2_plan_launch(self.handle)
+        self.solver._after()
+        _check(code)
 
     def close(self):
         if self.handle:

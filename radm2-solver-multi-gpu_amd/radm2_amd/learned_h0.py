@@ -3,26 +3,28 @@
 A 96 -> 128 -> 128 -> 16 SiLU MLP (83 live features, 2 live outputs) runs on
 the project's BF16-MFMA kernels.  Output 0 is log10 of a proposed initial
 step h0, output 1 a confidence logit.  The FP64 solver always does the
-integration: a proposal only replaces the options' Hstart for cells whose
-confidence exceeds the threshold, clamped to [1e-6 s, dt].
+integration: a proposal only replaces the options' Hstart for the cells
+whose confidence exceeds the threshold (the solver treats it as that cell's
+user Hstart: min(h0, dt, Hmax)).
 
-Training data come from the solver itself, two runs over the same cells:
-  run A  plain call (INTEGRATE's Hstart = 0.01 dt): per cell the attempt
-         count and `hfirst`, the controller's step proposal after the first
-         accepted step, FacSafe/err^(1/3) times that step, uncapped -- an
-         estimate of the largest step the local error allows at t = 0;
-  run B  the same cells with h0 = clamp(hfirst, 1e-6, dt).
-Targets: log10 clamp(hfirst, 1e-6, dt) (cells that succeeded), and the label
-"run B took fewer attempts than run A" for the confidence.
+Training data come from the solver itself.  Every training cell is
+integrated once with the plain call (INTEGRATE's Hstart = 0.01 dt) and once
+per candidate h0 on a half-decade grid (1e-11 s .. 1 s); its attempt counts
+over the grid, smoothed over +-1/2 decade (a window of three candidates, so
+that a proposal off by the network's typical error still lands on a good
+value), give
+  target 0  log10 of the candidate with the fewest smoothed attempts;
+  target 1  1 if that smoothed minimum is below the plain call's count.
 """
 import numpy as np
 
 from . import Opts, mlp
 
 DIMS = [96, 128, 128, 16]
+GRID = tuple(np.arange(-11.0, 0.01, 0.5))
 
 
-def _cells_to_batch(cells, device="cuda", hfirst=True):
+def _cells_to_batch(cells, device="cuda", hfirst=False):
     from . import CellBatch, synth
     d = synth.to_device(cells, device)
     return CellBatch(d["conc"], d["temp"], d["cair"], d["h2o"], d["jrate"], rc_n2o5=d["rc_n2o5"],
@@ -41,33 +43,61 @@ def fit_normalizer(batch):
     return mu.contiguous(), sd.contiguous()
 
 
-def generate(solver, batch, dt, rtol=1e-3, atol=1.0, log=None):
-    """Solver-generated targets for every cell of `batch` (state restored)."""
+def sweep(solver, batch, dt, rtol=1e-3, atol=1.0, grid=GRID):
+    """Attempt counts of every cell: the plain call and each constant h0 of
+    `grid` (log10 s).  Returns (plain [n] int32, counts [len(grid) x n]
+    float32 with failed cells at +inf); the batch's state is restored."""
     import torch
     conc0 = batch.conc.clone()
-    n = batch.ncells
-    solver.integrate(batch, dt, rtol, atol)
-    solver.sync()
-    ns_a = batch.nstep.clone()
-    st_a = batch.status.clone()
-    hf = batch.hfirst.clone()
-    h_target = hf.clamp(1e-6, dt)
-    valid = (st_a == 1) & (hf > 0) & torch.isfinite(hf)
-    batch.conc.copy_(conc0)
     h0_saved = batch.h0
-    batch.h0 = h_target.contiguous()
+    n = batch.ncells
+    batch.h0 = None
     solver.integrate(batch, dt, rtol, atol)
-    solver.sync()
-    ns_b = batch.nstep.clone()
+    plain = torch.where(batch.status == 1, batch.nstep, torch.iinfo(torch.int32).max)
+    counts = torch.empty((len(grid), n), dtype=torch.float32, device=conc0.device)
+    h0 = torch.empty(n, dtype=torch.float64, device=conc0.device)
+    batch.h0 = h0
+    for i, lh in enumerate(grid):
+        batch.conc.copy_(conc0)
+        h0.fill_(10.0 ** lh)
+        solver.integrate(batch, dt, rtol, atol)
+        counts[i] = torch.where(batch.status == 1, batch.nstep.float(),
+                                torch.full_like(counts[i], float("inf")))
     batch.h0 = h0_saved
     batch.conc.copy_(conc0)
-    tgt = torch.stack([torch.log10(h_target), (ns_b < ns_a).to(torch.float64)], 1).float()
+    solver.sync()
+    return plain, counts
+
+
+def targets(plain, counts, grid=GRID):
+    """Regression and confidence targets from a sweep (see module doc)."""
+    import torch
+    c = counts
+    pad = torch.cat([c[:1], c, c[-1:]])
+    smooth = (pad[:-2] + pad[1:-1] + pad[2:]) / 3.0
+    best = smooth.argmin(0)
+    g = torch.tensor(grid, dtype=torch.float32, device=c.device)
+    sbest = smooth.gather(0, best[None])[0]
+    valid = torch.isfinite(sbest) & (plain < torch.iinfo(torch.int32).max)
+    label = (sbest < plain.float()).float()
+    tgt = torch.stack([g[best], label], 1)
     mask = torch.stack([valid, valid], 1).float()
+    return tgt.contiguous(), mask.contiguous()
+
+
+def generate(solver, batch, dt, rtol=1e-3, atol=1.0, log=None):
+    """Solver-generated targets for every cell of `batch` (state restored)."""
+    plain, counts = sweep(solver, batch, dt, rtol, atol)
+    tgt, mask = targets(plain, counts)
+    info = dict(plain=plain.double().mean().item(),
+                per_h0={"%.1f" % g: counts[i].double().mean().item() for i, g in enumerate(GRID)},
+                cell_best=counts.min(0).values.double().mean().item())
     if log:
-        log("targets: %d cells, valid %.4f, helped %.4f, attempts plain %.2f -> oracle h0 %.2f"
-            % (n, valid.double().mean().item(), tgt[:, 1].double().mean().item(),
-               ns_a.double().mean().item(), ns_b.double().mean().item()))
-    return tgt.contiguous(), mask.contiguous(), dict(nstep_plain=ns_a, nstep_oracle_h0=ns_b)
+        log("targets: %d cells, valid %.4f, helped %.4f, attempts plain %.2f, best constant h0 "
+            "%.2f, per-cell best %.2f" % (batch.ncells, mask[:, 0].double().mean().item(),
+                                          tgt[:, 1].double().mean().item(), info["plain"],
+                                          min(info["per_h0"].values()), info["cell_best"]))
+    return tgt, mask, info
 
 
 def train(net, X, tgt, mask, epochs=3, batch=8192, lr=2e-3, log=None, seed=0):
@@ -87,7 +117,7 @@ def train(net, X, tgt, mask, epochs=3, batch=8192, lr=2e-3, log=None, seed=0):
             xb, tb, mb = X[idx].contiguous(), tgt[idx].contiguous(), mask[idx].contiguous()
             out = net.forward(xb, keep=True)
             dout = torch.empty_like(out)
-            mlp._ck(mlp.lib().radm2_mlp_h0_loss(None, batch, out.shape[1], mlp._p(out), mlp._p(tb),
+            mlp._ck(mlp.lib().radm2_mlp_h0_loss(mlp._st(), batch, out.shape[1], mlp._p(out), mlp._p(tb),
                                                 mlp._p(mb), mlp._p(dout), mlp._p(loss)))
             net.zero_grad()
             net.backward(dout)
@@ -110,7 +140,7 @@ def propose(net, batch, mu, sd, dt, threshold=0.5, ncells=None, fallback=0.0):
     out = net.forward(X)
     h0 = torch.empty(n, dtype=torch.float64, device=X.device)
     used = torch.empty(n, dtype=torch.int32, device=X.device)
-    mlp._ck(mlp.lib().radm2_mlp_h0_select(None, n, out.shape[1], mlp._p(out), float(dt),
+    mlp._ck(mlp.lib().radm2_mlp_h0_select(mlp._st(), n, out.shape[1], mlp._p(out), float(dt),
                                           float(threshold), float(fallback), mlp._p(h0),
                                           mlp._p(used)))
     return h0, used
@@ -193,7 +223,7 @@ def main():
 
     from . import Solver, synth
     ap = argparse.ArgumentParser(description="train the learned-h0 network on solver data")
-    ap.add_argument("--train-cells", type=int, default=2097152)
+    ap.add_argument("--train-cells", type=int, default=1048576)
     ap.add_argument("--eval-cells", type=int, default=1048576)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.dirname(
@@ -209,6 +239,7 @@ def main():
     r = evaluate(s, net, mu, sd, ev)
     out = {k: {kk: vv for kk, vv in v.items() if kk not in ("conc", "nstep")} for k, v in r.items()}
     out["losses"] = losses
+    out["train_sweep"] = info
     print(json.dumps(out))
 
 

@@ -55,6 +55,13 @@ def _p(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
+def _st():
+    """torch's current stream: every kernel here is ordered with torch's own
+    operations on the same buffers."""
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
 def _ck(code):
     if code != 0:
         from . import Radm2Error, strerror
@@ -90,7 +97,7 @@ class Dense:
         self.sync_bf16()
 
     def sync_bf16(self):
-        _ck(lib().radm2_mlp_to_bf16(None, self.w.numel(), _p(self.w), _p(self.wb)))
+        _ck(lib().radm2_mlp_to_bf16(_st(), self.w.numel(), _p(self.w), _p(self.wb)))
 
     def load(self, w, b):
         import torch
@@ -128,11 +135,11 @@ class MLP:
             Z = torch.empty((M, L.N), dtype=torch.float32, device=self.device) if keep else None
             if last:
                 out = torch.empty((M, L.N), dtype=torch.float32, device=self.device)
-                _ck(lib().radm2_mlp_dense_fwd(None, M, L.K, L.N, _p(x), _p(L.wb), _p(L.b), L.act,
+                _ck(lib().radm2_mlp_dense_fwd(_st(), M, L.K, L.N, _p(x), _p(L.wb), _p(L.b), L.act,
                                               _p(Z), None, _p(out)))
             else:
                 yb = torch.empty((M, L.N), dtype=torch.int16, device=self.device)
-                _ck(lib().radm2_mlp_dense_fwd(None, M, L.K, L.N, _p(x), _p(L.wb), _p(L.b), L.act,
+                _ck(lib().radm2_mlp_dense_fwd(_st(), M, L.K, L.N, _p(x), _p(L.wb), _p(L.b), L.act,
                                               _p(Z), _p(yb), None))
                 x = yb
                 acts.append(yb)
@@ -151,9 +158,9 @@ class MLP:
             L = self.layers[i]
             dZb = torch.empty((M, L.N), dtype=torch.int16, device=self.device)
             dZf = torch.empty((M, L.N), dtype=torch.float32, device=self.device)
-            _ck(lib().radm2_mlp_act_bwd(None, M * L.N, _p(dY), _p(zs[i]), L.act, _p(dZb), _p(dZf)))
+            _ck(lib().radm2_mlp_act_bwd(_st(), M * L.N, _p(dY), _p(zs[i]), L.act, _p(dZb), _p(dZf)))
             dX = torch.empty((M, L.K), dtype=torch.float32, device=self.device) if i > 0 else None
-            _ck(lib().radm2_mlp_dense_bwd(None, M, L.K, L.N, _p(acts[i]), _p(L.wb), _p(dZb),
+            _ck(lib().radm2_mlp_dense_bwd(_st(), M, L.K, L.N, _p(acts[i]), _p(L.wb), _p(dZb),
                                           _p(dZf), _p(dX), _p(L.gw), _p(L.gb)))
             dY = dX
 
@@ -165,9 +172,9 @@ class MLP:
     def adam(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
         self.step += 1
         for L in self.layers:
-            _ck(lib().radm2_mlp_adam(None, L.w.numel(), _p(L.w), _p(L.wb), _p(L.gw), _p(L.mw),
+            _ck(lib().radm2_mlp_adam(_st(), L.w.numel(), _p(L.w), _p(L.wb), _p(L.gw), _p(L.mw),
                                      _p(L.vw), lr, b1, b2, eps, self.step, wd))
-            _ck(lib().radm2_mlp_adam(None, L.b.numel(), _p(L.b), None, _p(L.gb), _p(L.mb),
+            _ck(lib().radm2_mlp_adam(_st(), L.b.numel(), _p(L.b), None, _p(L.gb), _p(L.mb),
                                      _p(L.vb), lr, b1, b2, eps, self.step, 0.0))
 
     # -- persistence ----------------------------------------------------------
@@ -190,6 +197,6 @@ def features(batch, mu, sd, ncells=None, F=FPAD, want_f32=False):
     n = batch.ncells if ncells is None else ncells
     X = torch.empty((n, F), dtype=torch.int16, device=batch.conc.device)
     Xf = torch.empty((n, F), dtype=torch.float32, device=batch.conc.device) if want_f32 else None
-    _ck(lib().radm2_mlp_features(None, n, batch.ld, _p(batch.conc), _p(batch.temp), _p(batch.cair),
+    _ck(lib().radm2_mlp_features(_st(), n, batch.ld, _p(batch.conc), _p(batch.temp), _p(batch.cair),
                                  _p(batch.h2o), _p(batch.jrate), _p(mu), _p(sd), F, _p(X), _p(Xf)))
     return X, Xf

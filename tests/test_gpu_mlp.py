@@ -24,7 +24,7 @@ def test_mfma_dense_forward_layout():
     Wb = _t(mlp.bf16_bits(W).view(np.int16))
     out = torch.empty((M, N), dtype=torch.float32, device="cuda")
     bt = _t(b)  # keep every operand alive until the kernel has run
-    mlp._ck(mlp.lib().radm2_mlp_dense_fwd(None, M, K, N, mlp._p(Xb), mlp._p(Wb), mlp._p(bt),
+    mlp._ck(mlp.lib().radm2_mlp_dense_fwd(mlp._st(), M, K, N, mlp._p(Xb), mlp._p(Wb), mlp._p(bt),
                                           mlp.ACT_NONE, None, None, mlp._p(out)))
     torch.cuda.synchronize()
     ref = R.bf16(X) @ R.bf16(W) + b
@@ -52,7 +52,7 @@ def test_forward_backward_match_oracle():
     dout = torch.zeros((M, 16), dtype=torch.float32, device="cuda")
     loss = torch.zeros(1, dtype=torch.float32, device="cuda")
     tt, tm = _t(tgt), _t(mask)  # keep alive: a dropped temporary's block is reused at once
-    mlp._ck(mlp.lib().radm2_mlp_h0_loss(None, M, 16, mlp._p(out), mlp._p(tt), mlp._p(tm),
+    mlp._ck(mlp.lib().radm2_mlp_h0_loss(mlp._st(), M, 16, mlp._p(out), mlp._p(tt), mlp._p(tm),
                                         mlp._p(dout), mlp._p(loss)))
     torch.cuda.synchronize()
     rl, rd = R.h0_loss(out.cpu().numpy().astype(np.float64), tgt, mask)
@@ -79,7 +79,7 @@ def test_adam_matches_numpy():
     tw, tg = _t(w.copy()), _t(g)
     m1 = torch.zeros(1000, dtype=torch.float32, device="cuda")
     m2 = torch.zeros_like(m1)
-    mlp._ck(mlp.lib().radm2_mlp_adam(None, 1000, mlp._p(tw), None, mlp._p(tg), mlp._p(m1),
+    mlp._ck(mlp.lib().radm2_mlp_adam(mlp._st(), 1000, mlp._p(tw), None, mlp._p(tg), mlp._p(m1),
                                      mlp._p(m2), 1e-2, 0.9, 0.999, 1e-8, 1, 0.0))
     mh = 0.1 * g / 0.1
     vh = 0.001 * g * g / 0.001
@@ -124,7 +124,7 @@ def test_training_reduces_loss():
         out = net.forward(Xb, keep=True)
         dout = torch.zeros_like(out)
         loss = torch.zeros(1, dtype=torch.float32, device="cuda")
-        mlp._ck(mlp.lib().radm2_mlp_h0_loss(None, M, 16, mlp._p(out), mlp._p(tt), mlp._p(tm),
+        mlp._ck(mlp.lib().radm2_mlp_h0_loss(mlp._st(), M, 16, mlp._p(out), mlp._p(tt), mlp._p(tm),
                                             mlp._p(dout), mlp._p(loss)))
         net.zero_grad()
         net.backward(dout)

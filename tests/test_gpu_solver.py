@@ -171,8 +171,8 @@ def test_options_and_h0_match_oracle(solver, cells):
     assert code == 0 and (out["status"] == 1).all()
     assert _close(out["conc"], co).mean() >= 0.99
     assert (out["nstep"] == oo["nstep"]).mean() >= 0.99
-    # per-cell initial step proposal, clamped to [1e-6, Hmax]
-    h0 = np.geomspace(1e-9, 1e3, 300)
+    # per-cell Hstart: min(h0, dt, Hmax), DeltaMin below 10 roundoff
+    h0 = np.geomspace(1e-16, 1e3, 300)
     b = _batch(sub, h0=h0)
     solver.integrate(b, 600.0, 1e-3, 1.0)
     solver.sync()

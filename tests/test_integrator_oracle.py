@@ -157,3 +157,22 @@ def test_decomp_zero_diagonal_reports_row():
     G[d] = 0.0
     rc, _ = O.decomp(G)
     assert rc == 18
+
+
+@pytest.mark.parametrize("h", [1e-16, 2e-10, 0.37, 6.0, 5e3])
+def test_per_cell_h0_is_a_user_hstart(h):
+    # a per-cell h0 must act exactly as RPAR[2] = h0 would: Hstart =
+    # min(h0, dt) (mgpu.cu:3752), then H = min(Hstart, Hmax) with DeltaMin
+    # below 10 roundoff (mgpu.cu:3475-3477) -- no Hmin clamp on a user Hstart
+    from radm2_amd import synth
+    c = synth.make_cells(1, 5, 60.0, 0.0)
+    k = O.rconst(c["temp"][0], c["cair"][0], c["jrate"][:, 0], c["rc_n2o5"][0])
+    y0 = c["conc"][:, 0]
+    a = _cell(y0, k, 600.0, 1e-3, 1.0, _opts(hmin=1e-3, hstart=h), h0=0.0)
+    b = _cell(y0, k, 600.0, 1e-3, 1.0, _opts(hmin=1e-3), h0=h)
+    assert a[0] == b[0] == 1
+    assert a[2:] == b[2:]
+    np.testing.assert_array_equal(a[1], b[1])
+    if h < 1e-14:   # both fell back to DeltaMin = 1e-6
+        d = _cell(y0, k, 600.0, 1e-3, 1.0, _opts(hmin=1e-3), h0=1e-6)
+        assert d[2:] == b[2:]
