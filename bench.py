@@ -6,9 +6,16 @@ atol 1 molec/cm3, seed 0, zenith U(0,100) deg (10% night cells), 5% plume
 cells (NOx 20-500 ppb, VOC x10).  A "step" is one integration of every
 cell over dt (one cell-integration per cell), inputs resident in HBM.
 
-Paths timed: the one-shot grid kernel (one lane per cell for the whole
-integration) and the persistent work-stealing kernel; the JSON line reports
-the persistent path as `value`, with the grid path and the speed-up beside it.
+Paths timed, each over the same K steps after W warm-up steps:
+  grid        one-shot kernel, one lane per cell for the whole integration;
+  persistent  work-stealing kernel (the library default) -- `value`;
+  learned     the shipped learned-h0 network (features + BF16-MFMA MLP +
+              selection kernels, then the persistent solver), inside the
+              timed region; reported as a speed-up over `persistent`, with
+              proposals floored at 1e-12 s and, separately, at 1e-6 s;
+  end_to_end  one step from pinned host buffers: H2D, solve, D2H.
+`accuracy` compares the plain and learned end states of the first 16,384
+cells with the GPU FP64 path at rtol 1e-10, atol 1e-3 molec/cm3.
 `roofline` is the FP64 compute roofline of the solver kernel (the only
 kernel of a step apart from a 1-element counter reset): algorithmic flops
 per launch from the per-cell accepted/rejected counts, over the kernel's
@@ -34,13 +41,15 @@ PEAK_HBM_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cells", type=int, default=1048576)
     ap.add_argument("--dt", type=float, default=600.0)
     ap.add_argument("--rtol", type=float, default=1e-3)
     ap.add_argument("--atol", type=float, default=1.0)
-    ap.add_argument("--paths", default="grid,persistent")
+    ap.add_argument("--paths", default="grid,persistent,learned")
+    ap.add_argument("--acc-cells", type=int, default=16384,
+                    help="cells checked against the rtol-1e-10 GPU solve (0 = skip)")
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
@@ -50,20 +59,22 @@ def parse():
     return ap.parse_args()
 
 
-def run_path(solver, batch, conc0, sched, args, torch):
+def run_path(solver, batch, conc0, sched, args, torch, step=None):
+    """W warm-up steps, then K timed steps; `step` replaces the plain
+    integrate call (the learned path)."""
     import radm2_amd
     solver.set_schedule(sched, args.block, args.refill)
+    call = step or (lambda: solver.integrate(batch, args.dt, args.rtol, args.atol))
     for _ in range(args.warmup):
         batch.conc.copy_(conc0)
-        solver.integrate(batch, args.dt, args.rtol, args.atol)
+        call()
         solver.sync()
     kernel_ms = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         batch.conc.copy_(conc0)
-        torch.cuda.synchronize()
-        solver.integrate(batch, args.dt, args.rtol, args.atol)
+        call()
         solver.sync()
         kernel_ms.append(solver.elapsed_ms())
     torch.cuda.synchronize()
@@ -73,15 +84,39 @@ def run_path(solver, batch, conc0, sched, args, torch):
     st = batch.status
     return dict(wall_s=wall, kernel_ms=kernel_ms, nacc=int(nacc.sum()), nrej=int(nrej.sum()),
                 nstep_max=int(batch.nstep.max()), nstep_mean=float(batch.nstep.double().mean()),
+                nrej_mean=float(batch.nrej.double().mean()),
                 ok=int((st == radm2_amd.CELL_OK).sum()))
+
+
+def learned_step(solver, batch, args, hlo):
+    """One learned-h0 step: features -> MLP -> h0 selection -> solver."""
+    from radm2_amd import learned_h0
+    net, mu, sd = _net()
+
+    def step():
+        h0, used = learned_h0.propose(net, batch, mu, sd, args.dt, hlo=hlo)
+        batch.h0 = h0
+        solver.integrate(batch, args.dt, args.rtol, args.atol)
+        step.used = used
+    return step
+
+
+_NET = []
+
+
+def _net():
+    if not _NET:
+        from radm2_amd import learned_h0
+        _NET.append(learned_h0.load(os.path.join(PKG, "mech", "h0_mlp.npz")))
+    return _NET[0]
 
 
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(cells are independent, so the path shards by contiguous slices -- "
-              "see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "(cells are independent: the path would shard by contiguous slices of "
+              "cells, one per GPU)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
 
@@ -108,15 +143,28 @@ def main():
                                 dev["jrate"], rc_n2o5=dev["rc_n2o5"])
     solver = radm2_amd.Solver(0)
     results = {}
-    for name in args.paths.split(","):
-        sched = {"grid": radm2_amd.SCHED_GRID, "persistent": radm2_amd.SCHED_PERSISTENT}[name]
-        results[name] = run_path(solver, batch, conc0, sched, args, torch)
+    ends = {}
+    na = min(args.acc_cells, n)
+    names = args.paths.split(",")
+    if "learned" in names:
+        names = [x for x in names if x != "learned"] + ["learned", "learned_hlo1e-6"]
+    for name in names:
+        step = None
+        if name.startswith("learned"):
+            step = learned_step(solver, batch, args, 1e-6 if name.endswith("1e-6") else 1e-12)
+        sched = radm2_amd.SCHED_GRID if name == "grid" else radm2_amd.SCHED_PERSISTENT
+        results[name] = run_path(solver, batch, conc0, sched, args, torch, step)
+        batch.h0 = None
         r = results[name]
+        if step is not None:
+            r["used"] = float(step.used.double().mean())
+        ends[name] = batch.conc[:, :na].clone()
         print("[bench] %s: wall %.3f s for %d steps, kernel ms %s, ok %d/%d, attempts mean %.1f max %d"
               % (name, r["wall_s"], args.steps, ["%.1f" % x for x in r["kernel_ms"]], r["ok"], n,
                  r["nstep_mean"], r["nstep_max"]), file=sys.stderr, flush=True)
 
-    head = "persistent" if "persistent" in results else list(results)[0]
+    head = "persistent" if "persistent" in results else [k for k in results
+                                                          if not k.startswith("learned")][0]
     r = results[head]
     ms_per_step = 1000.0 * r["wall_s"] / args.steps
     value = n / (ms_per_step / 1000.0)
@@ -155,9 +203,87 @@ def main():
     }
     if "grid" in results and "persistent" in results:
         out["persistent_vs_grid"] = (results["grid"]["wall_s"] / results["persistent"]["wall_s"])
+    for name in ("learned", "learned_hlo1e-6"):
+        if name in results and head in results:
+            L, P = results[name], results[head]
+            out[name] = {"speedup_vs_plain": P["wall_s"] / L["wall_s"],
+                         "cells_per_s": n / (L["wall_s"] / args.steps),
+                         "attempts_mean": L["nstep_mean"], "plain_attempts_mean": P["nstep_mean"],
+                         "rejected_mean": L["nrej_mean"], "plain_rejected_mean": P["nrej_mean"],
+                         "proposals_used": L.get("used"), "cells_ok": L["ok"],
+                         "timed": "features + MLP + h0 selection + solver, per step",
+                         "h0_floor_s": 1e-6 if name.endswith("1e-6") else 1e-12}
+    out["end_to_end"] = end_to_end(solver, cells, args, torch)
+    if na > 0:
+        out["accuracy"] = accuracy(solver, dev, conc0, ends, na, args, torch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cells, args)
     print(json.dumps(out), flush=True)
+
+
+def end_to_end(solver, cells, args, torch):
+    """One step from pinned host buffers: H2D of the inputs, the solve, D2H
+    of concentrations and per-cell counters (a warm-up step first)."""
+    import radm2_amd
+    keys = ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")
+    host = {k: torch.from_numpy(np.ascontiguousarray(cells[k])).pin_memory() for k in keys}
+    dev = {k: torch.empty(host[k].shape, dtype=torch.float64, device="cuda") for k in keys}
+    back = torch.empty(host["conc"].shape, dtype=torch.float64).pin_memory()
+    cnt = torch.empty((4, host["temp"].shape[0]), dtype=torch.int32).pin_memory()
+    b = radm2_amd.CellBatch(dev["conc"], dev["temp"], dev["cair"], dev["h2o"], dev["jrate"],
+                            rc_n2o5=dev["rc_n2o5"])
+    solver.set_schedule(radm2_amd.SCHED_PERSISTENT, args.block, args.refill)
+    times = []
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in keys:
+            dev[k].copy_(host[k], non_blocking=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        solver.integrate(b, args.dt, args.rtol, args.atol)
+        solver.sync()
+        t2 = time.perf_counter()
+        back.copy_(b.conc, non_blocking=True)
+        for i, x in enumerate((b.nstep, b.nacc, b.nrej, b.status)):
+            cnt[i].copy_(x, non_blocking=True)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        times.append((t1 - t0, t2 - t1, t3 - t2, t3 - t0))
+    h2d, solve, d2h, tot = times[-1]
+    n = host["temp"].shape[0]
+    return {"cells_per_s": n / tot, "h2d_ms": 1e3 * h2d, "solve_ms": 1e3 * solve,
+            "d2h_ms": 1e3 * d2h, "note": "pinned host buffers; second of two runs"}
+
+
+def accuracy(solver, dev, conc0, ends, na, args, torch):
+    """Max and median relative error (species above 1e3 molec/cm3) of each
+    path's end state against the GPU FP64 path at rtol 1e-10, atol 1e-3 on
+    the first `na` cells."""
+    import radm2_amd
+    keys = ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")
+    sub = {k: dev[k][..., :na].clone() for k in keys}
+    for k in ("temp", "cair", "h2o", "rc_n2o5"):
+        sub[k] = sub[k].contiguous()
+    sub["conc"] = conc0[:, :na].contiguous().clone()
+    b = radm2_amd.CellBatch(sub["conc"], sub["temp"], sub["cair"], sub["h2o"],
+                            sub["jrate"].contiguous(), rc_n2o5=sub["rc_n2o5"])
+    solver.set_schedule(radm2_amd.SCHED_PERSISTENT, 0, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.integrate(b, args.dt, 1e-10, 1e-3)
+    solver.sync()
+    el = time.perf_counter() - t0
+    ref = b.conc
+    big = ref.abs() > 1e3
+    out = {"reference": "GPU FP64 path, rtol 1e-10, atol 1e-3, first %d cells (%.1f s, "
+                        "%d/%d cells ok)" % (na, el, int((b.status == 1).sum()), na)}
+    for name, c in ends.items():
+        e = torch.where(big, (c - ref).abs() / ref.abs().clamp_min(1e-300), torch.zeros_like(ref))
+        per_cell = e.max(0).values
+        out[name] = {"max_rel_err": float(e.max()), "median_cell_max_rel_err":
+                     float(per_cell.median())}
+    return out
 
 
 def traffic_bytes(r):

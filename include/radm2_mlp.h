@@ -50,9 +50,14 @@ int radm2_mlp_dense_bwd(void *stream, int M, int K, int N, const uint16_t *X, co
 int radm2_mlp_h0_loss(void *stream, int M, int N, const float *out, const float *tgt,
                       const float *mask, float *dout, float *loss);
 
-/* Mean squared error over the first `live` columns (surrogate heads). */
-int radm2_mlp_mse_loss(void *stream, int M, int N, int live, const float *out, const float *tgt,
-                       float *dout, float *loss);
+/* Squared error of `heads` prediction blocks of one [M x N] output against
+ * tgt [M x live]: head h is columns [h*width, h*width + live), weighted per
+ * row by mask[m*heads + h] (mask NULL = 1); dout = d(sum/(M*live))/d(out),
+ * 0 in padding columns; *loss += weighted sum.  live <= width,
+ * heads*width <= N. */
+int radm2_mlp_mse_loss(void *stream, int M, int N, int heads, int width, int live,
+                       const float *out, const float *tgt, const float *mask, float *dout,
+                       float *loss);
 
 /* Adam step (1-based `step`) on FP32 weights w, refreshing the BF16 copy wb
  * (nullable); wd = decoupled-from-nothing L2 weight decay added to g. */
@@ -68,10 +73,39 @@ int radm2_mlp_features(void *stream, int64_t n, int64_t ld, const double *conc,
                        const double *jrate, const float *mu, const float *sd, int F, uint16_t *X,
                        float *Xf);
 
-/* h0[p] = clamp(10^out[p,0], 1e-12, hmax) if sigmoid(out[p,1]) > threshold,
- * else fallback (0 = the options' Hstart); used[p] (nullable) = 1 if taken. */
-int radm2_mlp_h0_select(void *stream, int64_t n, int N, const float *out, double hmax,
-                        float threshold, double fallback, double *h0, int32_t *used);
+/* h0[p] = clamp(10^out[p,0], hlo, hmax) if sigmoid(out[p,1]) > threshold,
+ * else fallback (0 = the options' Hstart); used[p] (nullable) = 1 if taken.
+ * 0 < hlo <= hmax, else RADM2_E_ARG. */
+int radm2_mlp_h0_select(void *stream, int64_t n, int N, const float *out, double hlo,
+                        double hmax, float threshold, double fallback, double *h0,
+                        int32_t *used);
+
+/* Surrogate targets tgt[p*59 + s] = (log10 max(c1,floor) - log10 max(c0,floor)
+ * - tmu[s]) / tsd[s] (tmu, tsd both NULL = unstandardised); c0, c1 [59][ld]. */
+int radm2_mlp_dlog_targets(void *stream, int64_t n, int64_t ld, const double *c0, const double *c1,
+                           double floor_c, const float *tmu, const float *tsd, float *tgt);
+
+/* Surrogate judge over a two-head output [n x N] (head B at column `width`):
+ * flag[p] = 1 (cell left untouched for the FP64 re-solve) when the heads'
+ * largest disagreement in log10 space, over species predicted above cmin,
+ * exceeds thr, or when the cell falls in the audit sample (fraction
+ * `audit`, a hash of p and seed); otherwise conc[.][p] becomes the heads'
+ * mean prediction and flag[p] = 0.  dis[p] (nullable) = the disagreement. */
+int radm2_mlp_surrogate_apply(void *stream, int64_t n, int64_t ld, int N, int width,
+                              const float *out, const float *tmu, const float *tsd, double floor_c,
+                              double cmin, float thr, double audit, uint64_t seed, double *conc,
+                              int32_t *flag, float *dis);
+
+/* idx[0 .. *count) = indices of the cells with flag != 0 (unspecified order);
+ * *count is reset on the stream first. */
+int radm2_mlp_compact(void *stream, int64_t n, const int32_t *flag, int32_t *idx, int32_t *count);
+
+/* Column gather (dst[r][i] = src[r][idx[i]]) and scatter (dst[r][idx[i]] =
+ * src[r][i]) of `rows` SoA rows, i < m; elem = 4 (int32) or 8 (double). */
+int radm2_mlp_gather(void *stream, int elem, int rows, int64_t lds, const void *src, int64_t m,
+                     const int32_t *idx, int64_t ldd, void *dst);
+int radm2_mlp_scatter(void *stream, int elem, int rows, int64_t lds, const void *src, int64_t m,
+                      const int32_t *idx, int64_t ldd, void *dst);
 
 #ifdef __cplusplus
 }

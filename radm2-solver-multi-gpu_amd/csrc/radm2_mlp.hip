@@ -161,20 +161,26 @@ __global__ void k_h0_loss(int M, int N, const float* __restrict__ out, const flo
   if ((threadIdx.x & 63) == 0) atomicAdd(loss, l);
 }
 
-// Mean squared error over the first `live` columns of each row (surrogate
-// heads): dOut and the loss sum.
-__global__ void k_mse_loss(int M, int N, int live, const float* __restrict__ out,
-                           const float* __restrict__ tgt, float* __restrict__ dout,
+// Squared error of `heads` prediction blocks against one target row: head
+// h occupies columns [h*width, h*width + live) of each N-wide output row and
+// is weighted per row by mask[m*heads + h] (mask NULL = weight 1).  Writes
+// dOut = d(sum / (M*live))/d(out) (zero in padding columns) and adds the
+// weighted sum of squares to *loss.
+__global__ void k_mse_loss(int M, int N, int heads, int width, int live,
+                           const float* __restrict__ out, const float* __restrict__ tgt,
+                           const float* __restrict__ mask, float* __restrict__ dout,
                            float* __restrict__ loss) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.0f;
   if (i < (int64_t)M * N) {
     const int n = (int)(i % N);
     const int64_t m = i / N;
-    if (n < live) {
-      const float e = out[i] - tgt[m * live + n];
-      dout[i] = 2.0f * e / ((float)M * live);
-      l = e * e;
+    const int h = n / width, c = n % width;
+    if (h < heads && c < live) {
+      const float w = mask ? mask[m * heads + h] : 1.0f;
+      const float e = out[i] - tgt[m * live + c];
+      dout[i] = 2.0f * e * w / ((float)M * live);
+      l = e * e * w;
     } else {
       dout[i] = 0.0f;
     }
@@ -227,19 +233,113 @@ __global__ void k_features(int64_t n, int64_t ld, const double* __restrict__ con
 }
 
 // Learned initial step from the network outputs: use 10^out[0] clamped to
-// [1e-12, hmax] when sigmoid(out[1]) > threshold, else `fallback` (0 selects
+// [hlo, hmax] when sigmoid(out[1]) > threshold, else `fallback` (0 selects
 // the options' Hstart in the solver).
-__global__ void k_h0_select(int64_t n, int N, const float* __restrict__ out, double hmax,
-                            float threshold, double fallback, double* __restrict__ h0,
-                            int32_t* __restrict__ used) {
+__global__ void k_h0_select(int64_t n, int N, const float* __restrict__ out, double hlo,
+                            double hmax, float threshold, double fallback,
+                            double* __restrict__ h0, int32_t* __restrict__ used) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const float conf = 1.0f / (1.0f + __expf(-out[p * N + 1]));
   const bool use = conf > threshold;
   double h = pow(10.0, (double)out[p * N]);
-  h = fmin(fmax(h, 1.0e-12), hmax);
+  h = fmin(fmax(h, hlo), hmax);
   h0[p] = use ? h : fallback;
   if (used) used[p] = use ? 1 : 0;
+}
+
+// Surrogate targets: per cell and species, the change of log10 max(c, floor)
+// over the integration, standardised as (d - tmu[s]) / tsd[s] (tmu/tsd NULL =
+// raw).  c0, c1: [59][ld] SoA; tgt: [n x 59] row-major.
+__global__ void k_dlog_targets(int64_t n, int64_t ld, const double* __restrict__ c0,
+                               const double* __restrict__ c1, double floor_c,
+                               const float* __restrict__ tmu, const float* __restrict__ tsd,
+                               float* __restrict__ tgt) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int s = 0; s < 59; s++) {
+    const double d = log10(fmax(c1[s * ld + p], floor_c)) - log10(fmax(c0[s * ld + p], floor_c));
+    float v = (float)d;
+    if (tmu) v = (v - tmu[s]) / tsd[s];
+    tgt[p * 59 + s] = v;
+  }
+}
+
+// 64-bit mix (splitmix64 finaliser) -> uniform [0, 1) for the audit draw.
+__device__ __forceinline__ double audit_u01(uint64_t p, uint64_t seed) {
+  uint64_t z = p + seed * 0x9E3779B97F4A7C15ull + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Surrogate judge, per cell: head predictions dA, dB of the log10 change of
+// each species (de-standardised); the heads' disagreement is the largest
+// |dA - dB| over species whose predicted end value exceeds cmin.  A cell is
+// flagged for the FP64 re-solve when the disagreement exceeds thr or when
+// it falls in the audit sample (fraction audit, drawn from a hash of the
+// cell index and seed); otherwise its concentrations are replaced by the
+// heads' mean prediction.  Flagged cells keep their input state.
+__global__ void k_surrogate_apply(int64_t n, int64_t ld, int N, int width,
+                                  const float* __restrict__ out, const float* __restrict__ tmu,
+                                  const float* __restrict__ tsd, double floor_c, double cmin,
+                                  float thr, double audit, uint64_t seed,
+                                  double* __restrict__ conc, int32_t* __restrict__ flag,
+                                  float* __restrict__ dis) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const float* o = out + p * N;
+  float worst = 0.0f;
+  for (int s = 0; s < 59; s++) {
+    const float a = o[s] * tsd[s] + tmu[s], b = o[width + s] * tsd[s] + tmu[s];
+    const double l0 = log10(fmax(conc[s * ld + p], floor_c));
+    if (l0 + (double)fmaxf(a, b) > log10(cmin)) worst = fmaxf(worst, fabsf(a - b));
+  }
+  const bool f = !(worst <= thr) || audit_u01((uint64_t)p, seed) < audit;
+  flag[p] = f ? 1 : 0;
+  if (dis) dis[p] = worst;
+  if (f) return;
+  for (int s = 0; s < 59; s++) {
+    const float a = o[s] * tsd[s] + tmu[s], b = o[width + s] * tsd[s] + tmu[s];
+    const double l0 = log10(fmax(conc[s * ld + p], floor_c));
+    conc[s * ld + p] = pow(10.0, l0 + 0.5 * ((double)a + (double)b));
+  }
+}
+
+__global__ void k_zero_i32(int32_t* __restrict__ x) { *x = 0; }
+
+// idx[0 .. *count) = the cells with flag != 0, in unspecified order
+// (wave-aggregated: one atomic per wave).
+__global__ void k_compact(int64_t n, const int32_t* __restrict__ flag, int32_t* __restrict__ idx,
+                          int32_t* __restrict__ count) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool f = p < n && flag[p] != 0;
+  const uint64_t mask = __ballot(f);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0 && mask) base = atomicAdd(count, (int32_t)__popcll(mask));
+  base = __shfl(base, 0);
+  if (f) idx[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int32_t)p;
+}
+
+// Column gather/scatter over SoA arrays of `rows` rows: gather dst[r][i] =
+// src[r][idx[i]], scatter dst[r][idx[i]] = src[r][i], for i < m.
+template <typename T>
+__global__ void k_gather(int rows, int64_t lds, const T* __restrict__ src, int64_t m,
+                         const int32_t* __restrict__ idx, int64_t ldd, T* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int64_t c = idx[i];
+  for (int r = 0; r < rows; r++) dst[r * ldd + i] = src[r * lds + c];
+}
+template <typename T>
+__global__ void k_scatter(int rows, int64_t lds, const T* __restrict__ src, int64_t m,
+                          const int32_t* __restrict__ idx, int64_t ldd, T* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int64_t c = idx[i];
+  for (int r = 0; r < rows; r++) dst[r * ldd + c] = src[r * lds + i];
 }
 
 }  // namespace mlp
@@ -293,11 +393,14 @@ int radm2_mlp_h0_loss(void* stream, int M, int N, const float* out, const float*
   return launch_ok();
 }
 
-int radm2_mlp_mse_loss(void* stream, int M, int N, int live, const float* out, const float* tgt,
-                       float* dout, float* loss) {
-  if (M <= 0 || N <= 0 || live <= 0 || live > N || !out || !tgt || !dout || !loss) return RADM2_E_ARG;
+int radm2_mlp_mse_loss(void* stream, int M, int N, int heads, int width, int live,
+                       const float* out, const float* tgt, const float* mask, float* dout,
+                       float* loss) {
+  if (M <= 0 || N <= 0 || heads <= 0 || width <= 0 || live <= 0 || live > width ||
+      (int64_t)heads * width > N || !out || !tgt || !dout || !loss)
+    return RADM2_E_ARG;
   hipLaunchKernelGGL(k_mse_loss, g1((int64_t)M * N, 256), dim3(256), 0, (hipStream_t)stream, M, N,
-                     live, out, tgt, dout, loss);
+                     heads, width, live, out, tgt, mask, dout, loss);
   return launch_ok();
 }
 
@@ -327,11 +430,71 @@ int radm2_mlp_features(void* stream, int64_t n, int64_t ld, const double* conc, 
   return launch_ok();
 }
 
-int radm2_mlp_h0_select(void* stream, int64_t n, int N, const float* out, double hmax,
-                        float threshold, double fallback, double* h0, int32_t* used) {
-  if (n <= 0 || N < 2 || !out || !h0 || !(hmax > 0.0)) return RADM2_E_ARG;
-  hipLaunchKernelGGL(k_h0_select, g1(n, 256), dim3(256), 0, (hipStream_t)stream, n, N, out, hmax,
-                     threshold, fallback, h0, used);
+int radm2_mlp_h0_select(void* stream, int64_t n, int N, const float* out, double hlo,
+                        double hmax, float threshold, double fallback, double* h0,
+                        int32_t* used) {
+  if (n <= 0 || N < 2 || !out || !h0 || !(hlo > 0.0) || !(hmax >= hlo)) return RADM2_E_ARG;
+  hipLaunchKernelGGL(k_h0_select, g1(n, 256), dim3(256), 0, (hipStream_t)stream, n, N, out, hlo,
+                     hmax, threshold, fallback, h0, used);
+  return launch_ok();
+}
+
+int radm2_mlp_dlog_targets(void* stream, int64_t n, int64_t ld, const double* c0, const double* c1,
+                           double floor_c, const float* tmu, const float* tsd, float* tgt) {
+  if (n <= 0 || ld < n || !c0 || !c1 || !tgt || !(floor_c > 0.0) || (!tmu != !tsd))
+    return RADM2_E_ARG;
+  hipLaunchKernelGGL(k_dlog_targets, g1(n, 128), dim3(128), 0, (hipStream_t)stream, n, ld, c0, c1,
+                     floor_c, tmu, tsd, tgt);
+  return launch_ok();
+}
+
+int radm2_mlp_surrogate_apply(void* stream, int64_t n, int64_t ld, int N, int width,
+                              const float* out, const float* tmu, const float* tsd, double floor_c,
+                              double cmin, float thr, double audit, uint64_t seed, double* conc,
+                              int32_t* flag, float* dis) {
+  if (n <= 0 || ld < n || width < 59 || 2 * width > N || !out || !tmu || !tsd || !conc || !flag ||
+      !(floor_c > 0.0) || !(cmin > 0.0) || !(thr >= 0.0f) || !(audit >= 0.0 && audit <= 1.0))
+    return RADM2_E_ARG;
+  hipLaunchKernelGGL(k_surrogate_apply, g1(n, 128), dim3(128), 0, (hipStream_t)stream, n, ld, N,
+                     width, out, tmu, tsd, floor_c, cmin, thr, audit, seed, conc, flag, dis);
+  return launch_ok();
+}
+
+int radm2_mlp_compact(void* stream, int64_t n, const int32_t* flag, int32_t* idx, int32_t* count) {
+  if (n <= 0 || n > INT32_MAX || !flag || !idx || !count) return RADM2_E_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, count);
+  hipLaunchKernelGGL(k_compact, g1(n, 256), dim3(256), 0, st, n, flag, idx, count);
+  return launch_ok();
+}
+
+int radm2_mlp_gather(void* stream, int elem, int rows, int64_t lds, const void* src, int64_t m,
+                     const int32_t* idx, int64_t ldd, void* dst) {
+  if (rows <= 0 || m < 0 || ldd < m || !src || !dst || (m > 0 && !idx) || (elem != 4 && elem != 8))
+    return RADM2_E_ARG;
+  if (m == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (elem == 8)
+    hipLaunchKernelGGL(k_gather<double>, g1(m, 256), dim3(256), 0, st, rows, lds,
+                       (const double*)src, m, idx, ldd, (double*)dst);
+  else
+    hipLaunchKernelGGL(k_gather<int32_t>, g1(m, 256), dim3(256), 0, st, rows, lds,
+                       (const int32_t*)src, m, idx, ldd, (int32_t*)dst);
+  return launch_ok();
+}
+
+int radm2_mlp_scatter(void* stream, int elem, int rows, int64_t lds, const void* src, int64_t m,
+                      const int32_t* idx, int64_t ldd, void* dst) {
+  if (rows <= 0 || m < 0 || lds < m || !src || !dst || (m > 0 && !idx) || (elem != 4 && elem != 8))
+    return RADM2_E_ARG;
+  if (m == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (elem == 8)
+    hipLaunchKernelGGL(k_scatter<double>, g1(m, 256), dim3(256), 0, st, rows, lds,
+                       (const double*)src, m, idx, ldd, (double*)dst);
+  else
+    hipLaunchKernelGGL(k_scatter<int32_t>, g1(m, 256), dim3(256), 0, st, rows, lds,
+                       (const int32_t*)src, m, idx, ldd, (int32_t*)dst);
   return launch_ok();
 }
 

@@ -129,8 +129,12 @@ def train(net, X, tgt, mask, epochs=3, batch=8192, lr=2e-3, log=None, seed=0):
     return losses
 
 
-def propose(net, batch, mu, sd, dt, threshold=0.5, ncells=None, fallback=0.0):
-    """Features -> MLP -> h0 per cell (device tensor) and the taken mask."""
+HLO = 1e-12   # lowest proposal passed on (the solver's own floor is 10 roundoff)
+
+
+def propose(net, batch, mu, sd, dt, threshold=0.5, ncells=None, fallback=0.0, hlo=HLO):
+    """Features -> MLP -> h0 per cell (device tensor) and the taken mask;
+    proposals are clamped to [hlo, dt]."""
     import torch
     n = batch.ncells if ncells is None else ncells
     X, _ = mlp.features(batch, mu, sd, ncells=n)
@@ -140,9 +144,9 @@ def propose(net, batch, mu, sd, dt, threshold=0.5, ncells=None, fallback=0.0):
     out = net.forward(X)
     h0 = torch.empty(n, dtype=torch.float64, device=X.device)
     used = torch.empty(n, dtype=torch.int32, device=X.device)
-    mlp._ck(mlp.lib().radm2_mlp_h0_select(mlp._st(), n, out.shape[1], mlp._p(out), float(dt),
-                                          float(threshold), float(fallback), mlp._p(h0),
-                                          mlp._p(used)))
+    mlp._ck(mlp.lib().radm2_mlp_h0_select(mlp._st(), n, out.shape[1], mlp._p(out), float(hlo),
+                                          float(dt), float(threshold), float(fallback),
+                                          mlp._p(h0), mlp._p(used)))
     return h0, used
 
 
@@ -178,7 +182,7 @@ def load(path, device="cuda"):
 
 
 def evaluate(solver, net, mu, sd, cells, dt=600.0, rtol=1e-3, atol=1.0, threshold=0.5,
-             reps=1):
+             reps=1, hlo=HLO):
     """Plain vs learned-h0 on the same cells: attempts, device time (solver
     launch, plus feature/MLP/selection time for the learned path) and the
     final states of both runs (host arrays)."""
@@ -197,7 +201,7 @@ def evaluate(solver, net, mu, sd, cells, dt=600.0, rtol=1e-3, atol=1.0, threshol
             t0 = time.perf_counter()
             used = None
             if name == "learned":
-                h0, used = propose(net, b, mu, sd, dt, threshold)
+                h0, used = propose(net, b, mu, sd, dt, threshold, hlo=hlo)
                 b.h0 = h0
             solver.integrate(b, dt, rtol, atol)
             solver.sync()

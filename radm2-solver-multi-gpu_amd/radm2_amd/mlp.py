@@ -26,14 +26,25 @@ _SIGS = {
     "radm2_mlp_act_bwd": [_P, ctypes.c_int64, _P, _P, ctypes.c_int, _P, _P],
     "radm2_mlp_dense_bwd": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P, _P, _P],
     "radm2_mlp_h0_loss": [_P, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P],
-    "radm2_mlp_mse_loss": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P],
+    "radm2_mlp_mse_loss": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           _P, _P, _P, _P, _P],
     "radm2_mlp_adam": [_P, ctypes.c_int64, _P, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float,
                        ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float],
     "radm2_mlp_to_bf16": [_P, ctypes.c_int64, _P, _P],
     "radm2_mlp_features": [_P, ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P,
                            ctypes.c_int, _P, _P],
-    "radm2_mlp_h0_select": [_P, ctypes.c_int64, ctypes.c_int, _P, ctypes.c_double, ctypes.c_float,
-                            ctypes.c_double, _P, _P],
+    "radm2_mlp_h0_select": [_P, ctypes.c_int64, ctypes.c_int, _P, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_float, ctypes.c_double, _P, _P],
+    "radm2_mlp_dlog_targets": [_P, ctypes.c_int64, ctypes.c_int64, _P, _P, ctypes.c_double, _P, _P,
+                               _P],
+    "radm2_mlp_surrogate_apply": [_P, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int, _P,
+                                  _P, _P, ctypes.c_double, ctypes.c_double, ctypes.c_float,
+                                  ctypes.c_double, ctypes.c_uint64, _P, _P, _P],
+    "radm2_mlp_compact": [_P, ctypes.c_int64, _P, _P, _P],
+    "radm2_mlp_gather": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _P, ctypes.c_int64, _P,
+                         ctypes.c_int64, _P],
+    "radm2_mlp_scatter": [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int64, _P, ctypes.c_int64, _P,
+                          ctypes.c_int64, _P],
 }
 EXPORTED_SYMBOLS = tuple(_SIGS)
 _lib = None

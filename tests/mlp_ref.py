@@ -63,3 +63,18 @@ def h0_loss(out, tgt, mask):
     d[:, 0] = 2 * e * mask[:, 0] / M
     d[:, 1] = (p - t) * mask[:, 1] / M
     return loss, d
+
+
+def mse_heads(out, tgt, heads, width, mask=None):
+    """Squared error of `heads` column blocks of out against tgt [M x live],
+    weighted per row and head by mask [M x heads]; returns (sum, dout) with
+    dout = d(sum / (M*live))/d(out)."""
+    M, live = tgt.shape
+    d = np.zeros_like(out)
+    total = 0.0
+    for h in range(heads):
+        w = np.ones(M) if mask is None else mask[:, h]
+        e = out[:, h * width:h * width + live] - tgt
+        total += np.sum(e * e * w[:, None])
+        d[:, h * width:h * width + live] = 2 * e * w[:, None] / (M * live)
+    return total, d

@@ -102,3 +102,25 @@ def test_integrate_defaults_resolve_like_integrate_call():
     assert (o.hmin, o.hmax, o.facmin, o.facmax, o.facrej, o.facsafe, o.max_steps) == \
         (0.0, 600.0, 0.2, 6.0, 0.1, 0.9, 100000)
     assert o.roundoff == 2.0 ** -52
+
+
+def test_learned_part_argument_checks_need_no_device():
+    # every call here is rejected before any launch (pointers are never read)
+    from radm2_amd import mlp
+    L = mlp.lib()
+    P = ctypes.c_void_p(16)
+    E = -101
+    assert L.radm2_mlp_h0_select(None, 16, 16, P, 0.0, 600.0, 0.5, 0.0, P, None) == E     # hlo <= 0
+    assert L.radm2_mlp_h0_select(None, 16, 16, P, 1.0, 0.5, 0.5, 0.0, P, None) == E       # hmax < hlo
+    assert L.radm2_mlp_h0_select(None, 16, 1, P, 1e-9, 600.0, 0.5, 0.0, P, None) == E     # N < 2
+    assert L.radm2_mlp_mse_loss(None, 16, 128, 2, 64, 65, P, P, None, P, P) == E           # live > width
+    assert L.radm2_mlp_mse_loss(None, 16, 96, 2, 64, 59, P, P, None, P, P) == E            # heads*width > N
+    assert L.radm2_mlp_surrogate_apply(None, 16, 16, 128, 58, P, P, P, 1.0, 1e3, 1e-3, 0.05, 0,
+                                       P, P, None) == E                                     # width < 59
+    assert L.radm2_mlp_surrogate_apply(None, 16, 16, 128, 64, P, P, P, 1.0, 1e3, 1e-3, 1.5, 0,
+                                       P, P, None) == E                                     # audit > 1
+    assert L.radm2_mlp_compact(None, 0, P, P, P) == E
+    assert L.radm2_mlp_gather(None, 2, 1, 16, P, 4, P, 4, P) == E                          # elem size
+    assert L.radm2_mlp_scatter(None, 8, 1, 2, P, 4, P, 16, P) == E                         # lds < m
+    assert L.radm2_mlp_gather(None, 8, 1, 16, P, 0, None, 0, P) == 0                       # empty: no-op
+    assert L.radm2_mlp_dlog_targets(None, 16, 16, P, P, 1.0, P, None, P) == E              # tmu w/o tsd

@@ -48,3 +48,17 @@ def test_parity_config0_subset(solver, schedule):
     # exp/pow differ by an ulp, which can flip a rare accept/reject decision
     assert ok.mean() >= 0.99
     assert same.mean() >= 0.99
+
+
+def test_parity_tight_tolerance_reference_run(solver):
+    # the GPU FP64 path at rtol 1e-10, atol 1e-3 serves as the accuracy
+    # reference over many cells; it must agree with the oracle's own run
+    from radm2_amd import synth
+    cells = synth.make_cells(64, 11, 100.0, 0.05)
+    co, oo, code = O.integrate(cells, 600.0, 1e-10, 1e-3, nthreads=0)
+    assert code == 0
+    cg, og = _run_gpu(solver, cells, 600.0, 1e-10, 1e-3, 1)
+    assert (og["status"] == 1).all()
+    same, ok, rel = _compare(cg, og, co, oo, "rtol 1e-10")
+    # both are converged solutions: they agree far below the rtol-1e-3 error
+    assert rel.max() <= 1e-6

@@ -44,3 +44,25 @@ def test_bf16_rounding_is_nearest_even():
     assert y[1] == 1.0            # exactly half an ulp above 1.0: ties to even
     assert y[2] == 1.0078125      # above half: rounds up
     assert np.isfinite(y[3])
+
+
+def test_mse_heads_gradient_is_the_derivative():
+    rng = np.random.default_rng(3)
+    M, live, width, heads = 5, 3, 4, 2
+    out = rng.standard_normal((M, heads * width + 2))
+    tgt = rng.standard_normal((M, live))
+    mask = (rng.random((M, heads)) < 0.6).astype(float)
+    s, d = R.mse_heads(out, tgt, heads, width, mask)
+    h = 1e-6
+    for i in range(M):
+        for j in range(out.shape[1]):
+            o = out.copy()
+            o[i, j] += h
+            fd = (R.mse_heads(o, tgt, heads, width, mask)[0] - s) / h / (M * live)
+            assert abs(fd - d[i, j]) <= 1e-5 * (1 + abs(fd)), (i, j)
+    # padding columns and masked-out rows carry no gradient
+    assert np.all(d[:, live:width] == 0) and np.all(d[:, heads * width:] == 0)
+    for i in range(M):
+        for hh in range(heads):
+            if mask[i, hh] == 0:
+                assert np.all(d[i, hh * width:hh * width + live] == 0)
