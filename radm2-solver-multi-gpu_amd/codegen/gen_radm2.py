@@ -394,6 +394,199 @@ def gen_solve(mech):
     return "\n".join(out)
 
 
+def gen_solve_pf(mech, G):
+    """solve() with the LU streamed in groups of G entries, each group's
+    loads issued one group ahead of the multiply-adds that consume them
+    (scheduling fences keep the compiler from sinking the loads back to
+    their uses).  Same operations in the same order as solve(): a nested
+    fma(a, b, fma(c, d, x)) is written as two accumulating statements."""
+    lu = mech["lu"]
+    crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
+    ops = []          # (lu entry, statement using lu_<entry>)
+    for i in range(59):
+        for kk in range(crow[i], diag[i]):
+            ops.append((kk, "x%d = fma(-lu%d, x%d, x%d);" % (i, kk, icol[kk], i)))
+    for i in range(58, -1, -1):
+        for kk in range(diag[i] + 1, crow[i + 1]):
+            ops.append((kk, "x%d = fma(-lu%d, x%d, x%d);" % (i, kk, icol[kk], i)))
+        ops.append((diag[i], "x%d = x%d * lu%d;" % (i, i, diag[i])))
+    assert sorted(e for e, _ in ops) == list(range(lu["nnz"]))
+    groups = [ops[a:a + G] for a in range(0, len(ops), G)]
+    fence = "  __builtin_amdgcn_sched_barrier(0);"
+    out = ["// solve() with the LU streamed %d entries at a time, one group ahead" % G]
+    out.append("template <class GA, class XA>")
+    out.append("__device__ __forceinline__ void solve_pf%d(GA g, XA x) {" % G)
+    for i in range(59):
+        out.append("  double x%d = RD(x, %d);" % (i, i))
+
+    def loads(grp):
+        return ["  const double lu%d = RD(g, %d);" % (e, e) for e, _ in grp]
+    out += loads(groups[0])
+    for m, grp in enumerate(groups):
+        if m + 1 < len(groups):
+            out += loads(groups[m + 1])
+        out.append(fence)
+        out += ["  " + st for _, st in grp]
+        out.append(fence)
+    for i in range(59):
+        out.append("  WR(x, %d, x%d);" % (i, i))
+    out.append("}")
+    return "\n".join(out)
+
+
+def schedule_loads(ops, loads, G):
+    """Body lines for `ops` (statement, [load indices], stored entry or None)
+    with the `loads` (name, source, producing entry or None) issued in
+    batches of G behind scheduling fences, so that between G and 2G loads
+    run ahead of their first use; a load waits for the op storing its
+    entry.  Returns (lines, number of empty batches)."""
+    store_op = {st: i for i, (_, _, st) in enumerate(ops) if st is not None}
+    fence = "  __builtin_amdgcn_sched_barrier(0);"
+    out = []
+    issued = 0
+    stalls = 0
+    for i, (st, used, _) in enumerate(ops):
+        nm = max(used) if used else -1
+        while issued <= nm or (issued - nm - 1 < G and issued < len(loads)):
+            batch = []
+            while len(batch) < G and issued < len(loads):
+                name, src, dep = loads[issued]
+                if dep is not None and store_op[dep] >= i:
+                    break            # its producer is not emitted yet
+                batch.append("  const double %s = %s;" % (name, src))
+                issued += 1
+            if not batch:
+                assert issued > nm
+                stalls += 1
+                break
+            out.append(fence)
+            out += batch
+            out.append(fence)
+        out.append("  " + st)
+    assert issued == len(loads)
+    return out, stalls
+
+
+def gen_fun_pf(mech, G, window=16):
+    """fun() with its loads scheduled like jac_decomp_pf(): the same
+    statements in the same order; rate constants are loaded once each,
+    concentrations once per window of reactions."""
+    by_r = [[] for _ in range(mech["nreact"])]
+    for i, terms in enumerate(mech["tendency"]):
+        for r, c in terms:
+            by_r[r].append((i, c))
+    loads, load_ix, ops = [], {}, []
+
+    def need(name, src):
+        if name not in load_ix:
+            load_ix[name] = len(loads)
+            loads.append((name, src, None))
+        return load_ix[name]
+    started = set(i for i in range(59) if not mech["tendency"][i])
+    for r, facs in enumerate(mech["monomials"]):
+        if not by_r[r]:
+            continue
+        used = [need("k%d" % r, "RD(k, %d)" % r)]
+        expr = "k%d" % r
+        for kind, i in facs:
+            if kind == "V":
+                nm = "v%d_%d" % (i, r // window)
+                used.append(need(nm, "RD(v, %d)" % i))
+                expr = "(%s * %s)" % (expr, nm)
+            else:
+                expr = "(%s * %s)" % (expr, factor_src(kind, i))
+        st = ["{ const double a = %s;" % expr]
+        for i, c in by_r[r]:
+            cv = float(c)
+            if i not in started:
+                rhs = "a" if cv == 1 else ("(-a)" if cv == -1 else "(%s * a)" % lit(c))
+                st.append("f%d = %s;" % (i, rhs))
+                started.add(i)
+            elif cv == 1:
+                st.append("f%d = f%d + a;" % (i, i))
+            elif cv == -1:
+                st.append("f%d = f%d - a;" % (i, i))
+            else:
+                st.append("f%d = fma(%s, a, f%d);" % (i, lit(c), i))
+        ops.append((" ".join(st) + " }", used, None))
+    out = ["// fun() with loads issued %d at a time ahead of their use" % G]
+    out.append("template <class KA, class VA, class FA>")
+    out.append("__device__ __forceinline__ void fun_pf%d(KA k, VA v, double h2o, double m, FA f) {" % G)
+    out.append("  double " + ", ".join("f%d" % i for i in range(59) if mech["tendency"][i]) + ";")
+    for i in range(59):
+        if not mech["tendency"][i]:
+            out.append("  double f%d = 0.0;" % i)
+    body, _ = schedule_loads(ops, loads, G)
+    out += body
+    for i in range(59):
+        out.append("  WR(f, %d, f%d);" % (i, i))
+    out.append("}")
+    return "\n".join(out)
+
+
+def gen_jac_decomp_pf(mech, G):
+    """jac_decomp() with every load scheduled explicitly: the same
+    operations in the same order, but rate constants, concentrations and
+    earlier LU rows are loaded in batches of G, issued so that between G and
+    2G loads run ahead of their first use (scheduling fences keep the
+    compiler from sinking them).  Within a row each rate constant and
+    concentration is loaded once.  A load of an LU entry is never issued
+    before the store that produces it."""
+    import re
+    lu = mech["lu"]
+    crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
+    terms = jac_terms(mech)
+    loads = []       # (name, source text, producing LU entry or None)
+    load_ix = {}
+    ops = []         # (statement, [load indices], stored LU entry or None)
+
+    def need(name, src, dep):
+        if name not in load_ix:
+            load_ix[name] = len(loads)
+            loads.append((name, src, dep))
+        return load_ix[name]
+
+    for k in range(59):
+        for kk in range(crow[k], crow[k + 1]):
+            c = icol[kk]
+            e = _entry_expr(terms[kk])
+            used = []
+            if e:
+                def sub(mo, used=used):
+                    arr, i = mo.group(1), int(mo.group(2))
+                    nm = "%s%d_%d" % (arr, i, k)
+                    used.append(need(nm, "RD(%s, %d)" % (arr, i), None))
+                    return nm
+                e = re.sub(r"RD\((k|v), (\d+)\)", sub, e)
+            if kk == diag[k]:
+                e = "(%s + ghinv)" % e if e else "ghinv"
+            ops.append(("double w%d_%d = %s;" % (c, k, e if e else "0.0"), used, None))
+        ops.append(("sing = (sing == 0 && w%d_%d == 0.0) ? %d : sing;" % (k, k, k + 1), [], None))
+        for kk in range(crow[k], diag[k]):
+            j = icol[kk]
+            li = need("u%d_%d" % (diag[j], k), "RD(g, %d)" % diag[j], diag[j])
+            ops.append(("w%d_%d = w%d_%d * u%d_%d;" % (j, k, j, k, diag[j], k), [li], None))
+            for jj in range(diag[j] + 1, crow[j + 1]):
+                c = icol[jj]
+                li = need("u%d_%d" % (jj, k), "RD(g, %d)" % jj, jj)
+                ops.append(("w%d_%d = fma(-w%d_%d, u%d_%d, w%d_%d);" % (c, k, j, k, jj, k, c, k),
+                            [li], None))
+        for kk in range(crow[k], crow[k + 1]):
+            c = icol[kk]
+            v = ("1.0 / w%d_%d" % (c, k)) if kk == diag[k] else ("w%d_%d" % (c, k))
+            ops.append(("WR(g, %d, %s);" % (kk, v), [], kk))
+    out = ["// jac_decomp() with loads issued %d at a time ahead of their use" % G]
+    out.append("template <class KA, class VA, class GA>")
+    out.append("__device__ __forceinline__ int jac_decomp_pf%d(KA k, VA v, double h2o, double m,"
+               " double ghinv, GA g) {" % G)
+    out.append("  int sing = 0;")
+    body, stalls = schedule_loads(ops, loads, G)
+    out += body
+    out.append("  return sing;")
+    out.append("}")
+    return "\n".join(out), len(loads), stalls
+
+
 def generate(dst=None, mech_path=None):
     mech = load_mech(mech_path)
     dst = dst or os.path.join(PKG, "csrc", "generated", "radm2_gen.h")
@@ -445,6 +638,14 @@ def generate(dst=None, mech_path=None):
         jdec,
         "",
         gen_solve(mech),
+        "",
+        gen_solve_pf(mech, 8),
+        "",
+        gen_solve_pf(mech, 16),
+        "",
+        gen_jac_decomp_pf(mech, 16)[0],
+        "",
+        gen_fun_pf(mech, 16),
         "",
         "}  // namespace gen",
         "}  // namespace radm2",

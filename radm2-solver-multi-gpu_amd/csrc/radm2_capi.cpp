@@ -72,11 +72,12 @@ hipError_t& last_hip_error() {
 constexpr int kDefaultGridBlock = 64;
 constexpr int kDefaultPersistentBlock = 64;
 // Defaults measured on MI355X at 1M cells (the autotuner may override):
-// persistent schedule, compiler-managed placement at >= 4 waves per SIMD,
-// refill as soon as 4 lanes of a wave are idle.
+// persistent schedule, explicit workspace placement with its load-scheduled
+// kernels at 1 wave per SIMD (all 512 registers of a lane), refill as soon
+// as 4 lanes of a wave are idle.
 constexpr int kDefaultRefillMin = 4;
-constexpr int kDefaultWaves = 4;
-constexpr int kDefaultPlacement = radm2::PLACE_PRIVATE;
+constexpr int kDefaultWaves = 1;
+constexpr int kDefaultPlacement = radm2::PLACE_WS;
 
 const radm2::KernelVariant* pick_variant(const radm2_ctx* c) {
   const int w = c->waves ? c->waves : kDefaultWaves;

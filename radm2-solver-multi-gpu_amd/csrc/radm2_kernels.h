@@ -58,8 +58,9 @@ constexpr int PLACE_PRIVATE = 0;  // compiler-managed (VGPR/AGPR, scratch spills
 constexpr int PLACE_WS = 1;       // explicit wave-interleaved HBM workspace
 
 // Kernel variants: one translation unit per (placement P, register budget
-// W = minimum resident waves per SIMD (1, 2, 4, 8), decomposition launder interval R);
-// block sizes 64 and 256 in each.
+// W = minimum resident waves per SIMD (1, 2, 4, 8), launder interval R of the
+// compiler-scheduled kernels, 0 in every built variant); block sizes 64 and
+// 256 in each.
 #define RADM2_DECLARE_VARIANT(P, W, R)                                                   \
   hipError_t launch_grid_p##P##_w##W##_r##R(int block, const LaunchArgs& a, hipStream_t st); \
   hipError_t launch_persistent_p##P##_w##W##_r##R(int block, int blocks, const LaunchArgs& a, \
@@ -70,8 +71,8 @@ RADM2_DECLARE_VARIANT(0, 2, 0)
 RADM2_DECLARE_VARIANT(0, 4, 0)
 RADM2_DECLARE_VARIANT(0, 8, 0)
 RADM2_DECLARE_VARIANT(1, 1, 0)
-RADM2_DECLARE_VARIANT(1, 2, 4)
-RADM2_DECLARE_VARIANT(1, 4, 2)
+RADM2_DECLARE_VARIANT(1, 2, 0)
+RADM2_DECLARE_VARIANT(1, 4, 0)
 #undef RADM2_DECLARE_VARIANT
 
 struct KernelVariant {

@@ -115,12 +115,74 @@ __device__ __forceinline__ void cell_attempt_top(CellState& s, const Ros3Opts& o
   }
 }
 
+// On workspace views the solves, the fused Jacobian/decomposition and the
+// tendency use the generated variants whose loads are issued 16 at a time
+// ahead of their use (the memory round trips of a lane overlap instead of
+// following one another); on private arrays the compiler keeps its own
+// schedule.  Both compute the same operations in the same order.
+template <class GA, class XA>
+__device__ __forceinline__ void stage_solve(GA g, XA x) {
+  if constexpr (__is_same(GA, WsView)) gen::solve_pf16(g, x);
+  else gen::solve(g, x);
+}
+
+template <int R, class KA, class YA, class GA>
+__device__ __forceinline__ int stage_jac_decomp(KA k, YA y, double h2o, double m, double ghinv,
+                                                GA g) {
+  if constexpr (__is_same(GA, WsView)) return gen::jac_decomp_pf16(k, y, h2o, m, ghinv, g);
+  else return gen::jac_decomp<R>(k, y, h2o, m, ghinv, g);
+}
+
+template <int RF, class KA, class VA, class FA>
+__device__ __forceinline__ void stage_fun(KA k, VA v, double h2o, double m, FA f) {
+  if constexpr (__is_same(FA, WsView)) gen::fun_pf16(k, v, h2o, m, f);
+  else gen::fun<RF>(k, v, h2o, m, f);
+}
+
+// Element-wise pass out[i] = fn(i, {in[0][i], .., in[N-1][i]}) over the 59
+// entries, CH at a time: all inputs of a chunk are loaded before any of its
+// outputs is stored, and the next chunk's inputs before this chunk's stores
+// (an output never aliases an input), so on workspace views a pass costs
+// about 59/CH memory round trips instead of one per entry.  fn runs for
+// i = 0, 1, .., 58 in that order.
+template <int CH, int N, class OA, class IA, class Fn>
+__device__ __forceinline__ void vmap(OA out, const IA (&in)[N], Fn fn) {
+  constexpr int NCH = (NVAR + CH - 1) / CH;
+  double buf[2][N][CH];
+#pragma unroll
+  for (int t = 0; t < N; t++)
+#pragma unroll
+    for (int j = 0; j < CH; j++)
+      if (j < NVAR) buf[0][t][j] = RD(in[t], j);
+#pragma unroll
+  for (int c = 0; c < NCH; c++) {
+    const int cur = c & 1;
+    if (c + 1 < NCH) {
+#pragma unroll
+      for (int t = 0; t < N; t++)
+#pragma unroll
+        for (int j = 0; j < CH; j++)
+          if ((c + 1) * CH + j < NVAR) buf[cur ^ 1][t][j] = RD(in[t], (c + 1) * CH + j);
+    }
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+      const int i = c * CH + j;
+      if (i < NVAR) {
+        double v[N];
+#pragma unroll
+        for (int t = 0; t < N; t++) v[t] = buf[cur][t][j];
+        WR(out, i, fn(i, v));
+      }
+    }
+  }
+}
+
 // One Ros3 attempt for a running cell whose attempt top has been applied.
 //   y   state (updated on acceptance)     k   rate constants
 //   g   659-entry LU workspace            f, k1, k2, k3: 59-entry vectors
-// R: launder interval of the fused Jacobian/decomposition (rows) and, as
-// 16 reactions per unit, of the tendency; phases are laundered apart
-// whenever the arrays are workspace views.
+// R: launder interval of the compiler-scheduled fused Jacobian/decomposition
+// (rows) and, as 16 reactions per unit, of the tendency on private arrays;
+// phases are laundered apart whenever the arrays are workspace views.
 template <int R, class KA, class YA, class GA, class VA>
 __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA k, YA y,
                                              double h2o, double m, GA g, VA f, VA k1, VA k2,
@@ -128,13 +190,13 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
   constexpr int RF = R > 0 ? 16 * R : 0;
   const double H = s.H;
   // F0 = f(y); K1 = G^-1 F0
-  gen::fun<RF>(k, y, h2o, m, k1);
+  stage_fun<RF>(k, y, h2o, m, k1);
   const double ghinv = 1.0 / (H * ROS_GAMMA1);
   launder(k);
   launder(y);
   launder(k1);
   // G = -J(y) + ghinv*I formed row by row inside its own LU decomposition
-  if (gen::jac_decomp<R>(k, y, h2o, m, ghinv, g) != 0) {
+  if (stage_jac_decomp<R>(k, y, h2o, m, ghinv, g) != 0) {
     s.status = ST_SINGULAR;
     return;
   }
@@ -149,61 +211,80 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
         // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
         launder(k1);
         launder(y);
-#pragma unroll
-        for (int i = 0; i < NVAR; i++) WR(k2, i, RD(y, i) + RD(k1, i));
+        {
+          const VA in[2] = {y, k1};
+          vmap<16>(k2, in, [](int, const double* a) { return a[0] + a[1]; });
+        }
         launder(k2);
         launder(k);
-        gen::fun<RF>(k, k2, h2o, m, f);
+        stage_fun<RF>(k, k2, h2o, m, f);
         launder(f);
         launder(k1);
-#pragma unroll
-        for (int i = 0; i < NVAR; i++) WR(k2, i, fma(RD(k1, i), hc21, RD(f, i)));
+        {
+          const VA in[2] = {k1, f};
+          vmap<16>(k2, in, [hc21](int, const double* a) { return fma(a[0], hc21, a[1]); });
+        }
         region = k2.region;
       } else if (st == 2) {
         // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
         launder(k2);
         launder(f);
         launder(k1);
-#pragma unroll
-        for (int i = 0; i < NVAR; i++) WR(k3, i, fma(RD(k2, i), hc32, fma(RD(k1, i), hc31, RD(f, i))));
+        {
+          const VA in[3] = {k2, k1, f};
+          vmap<16>(k3, in, [hc31, hc32](int, const double* a) {
+            return fma(a[0], hc32, fma(a[1], hc31, a[2]));
+          });
+        }
         region = k3.region;
       }
       WsView x = k1;
       x.region = region;
       launder(x);
       launder(g);
-      gen::solve(g, x);
+      stage_solve(g, x);
     }
   } else {
-    gen::solve(g, k1);
+    stage_solve(g, k1);
     // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
-#pragma unroll
-    for (int i = 0; i < NVAR; i++) WR(k2, i, RD(y, i) + RD(k1, i));
-    gen::fun<RF>(k, k2, h2o, m, f);
-#pragma unroll
-    for (int i = 0; i < NVAR; i++) WR(k2, i, fma(RD(k1, i), hc21, RD(f, i)));
-    gen::solve(g, k2);
+    {
+      const VA in[2] = {y, k1};
+      vmap<16>(k2, in, [](int, const double* a) { return a[0] + a[1]; });
+    }
+    stage_fun<RF>(k, k2, h2o, m, f);
+    {
+      const VA in[2] = {k1, f};
+      vmap<16>(k2, in, [hc21](int, const double* a) { return fma(a[0], hc21, a[1]); });
+    }
+    stage_solve(g, k2);
     // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
-#pragma unroll
-    for (int i = 0; i < NVAR; i++) WR(k3, i, fma(RD(k2, i), hc32, fma(RD(k1, i), hc31, RD(f, i))));
-    gen::solve(g, k3);
+    {
+      const VA in[3] = {k2, k1, f};
+      vmap<16>(k3, in, [hc31, hc32](int, const double* a) {
+        return fma(a[0], hc32, fma(a[1], hc31, a[2]));
+      });
+    }
+    stage_solve(g, k3);
   }
   launder(k1);
   launder(k2);
   launder(k3);
   launder(y);
-  // combination and WRMS error norm
+  // combination and WRMS error norm; Ynew kept in f until the decision
   double err = 0.0;
-#pragma unroll
-  for (int i = 0; i < NVAR; i++) {
-    const double a1 = RD(k1, i), a2 = RD(k2, i), a3 = RD(k3, i), yi = RD(y, i);
-    const double sm = fma(ROS_M3, a3, fma(ROS_M2, a2, ROS_M1 * a1));
-    const double yn = yi + sm;
-    const double e = fma(ROS_E3, a3, fma(ROS_E2, a2, ROS_E1 * a1));
-    const double ymax = dmax_ref(fabs(yi), fabs(yn));
-    const double q = e / fma(o.rtol, ymax, o.atol);
-    err = fma(q, q, err);
-    WR(f, i, yn);  // keep Ynew until the decision
+  {
+    const VA in[4] = {k1, k2, k3, y};
+    const double rtol = o.rtol, atol = o.atol;
+    vmap<8>(f, in, [&err, rtol, atol](int, const double* a) {
+      const double a1 = a[0], a2 = a[1], a3 = a[2], yi = a[3];
+      const double sm = fma(ROS_M3, a3, fma(ROS_M2, a2, ROS_M1 * a1));
+      const double yn = yi + sm;
+      const double e = fma(ROS_E3, a3, fma(ROS_E2, a2, ROS_E1 * a1));
+      const double ymax = dmax_ref(fabs(yi), fabs(yn));
+      const double q = e / fma(rtol, ymax, atol);
+      err = fma(q, q, err);
+      return yn;
+    });
   }
   err = sqrt(err / (double)NVAR);
   const double fsafe = dmax_ref(o.facmin, o.facsafe / pow(err, 1.0 / ROS_ELO));
@@ -213,8 +294,10 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
   if (err <= 1.0 || H <= o.hmin) {
     launder(f);
     launder(y);
-#pragma unroll
-    for (int i = 0; i < NVAR; i++) WR(y, i, RD(f, i));
+    {
+      const VA in[1] = {f};
+      vmap<16>(y, in, [](int, const double* a) { return a[0]; });
+    }
     s.T = s.T + H;
     if (s.na == 0) s.hfirst = H * fsafe;  // uncapped proposal (learned-h0 target)
     if (s.rejLast) Hnew = dmin_ref(Hnew, H);

@@ -28,7 +28,7 @@ from . import (PLACE_PRIVATE, PLACE_WORKSPACE, SCHED_GRID, SCHED_PERSISTENT, Cel
 
 KERNELS = [(PLACE_PRIVATE, 1), (PLACE_PRIVATE, 2), (PLACE_PRIVATE, 4), (PLACE_PRIVATE, 8),
            (PLACE_WORKSPACE, 1), (PLACE_WORKSPACE, 2), (PLACE_WORKSPACE, 4)]
-DEFAULT = dict(schedule=SCHED_PERSISTENT, placement=PLACE_PRIVATE, waves=4, block=64, refill=4)
+DEFAULT = dict(schedule=SCHED_PERSISTENT, placement=PLACE_WORKSPACE, waves=1, block=64, refill=4)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "radm2_amd", "autotune.json")
 
 
