@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--paths", default="grid,persistent,learned")
     ap.add_argument("--acc-cells", type=int, default=16384,
                     help="cells checked against the rtol-1e-10 GPU solve (0 = skip)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the end-to-end and accuracy legs (profiling runs)")
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
@@ -213,8 +215,9 @@ def main():
                          "proposals_used": L.get("used"), "cells_ok": L["ok"],
                          "timed": "features + MLP + h0 selection + solver, per step",
                          "h0_floor_s": 1e-6 if name.endswith("1e-6") else 1e-12}
-    out["end_to_end"] = end_to_end(solver, cells, args, torch)
-    if na > 0:
+    if not args.no_extra:
+        out["end_to_end"] = end_to_end(solver, cells, args, torch)
+    if na > 0 and not args.no_extra:
         out["accuracy"] = accuracy(solver, dev, conc0, ends, na, args, torch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cells, args)

@@ -587,6 +587,69 @@ def gen_jac_decomp_pf(mech, G):
     return "\n".join(out), len(loads), stalls
 
 
+def hot_ranking(mech):
+    """(array, index) entries of the rate constants (0), state (1) and LU (2)
+    ranked by memory accesses per Rosenbrock attempt of the load-scheduled
+    kernels, most accessed first: an LU entry is stored once and read by
+    the decomposition of every later row that eliminates with it and by the
+    three solves; rate constants and state entries count the loads of the
+    Jacobian/decomposition (once per row) and of the tendency (rate
+    constants twice, the state once), plus the state's three element-wise
+    passes.  Ties keep array order."""
+    import collections
+    import re
+    lu = mech["lu"]
+    crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
+    acc = collections.Counter()
+    for q in range(lu["nnz"]):
+        acc[(2, q)] += 4
+    for k in range(59):
+        for kk in range(crow[k], diag[k]):
+            j = icol[kk]
+            acc[(2, diag[j])] += 1
+            for jj in range(diag[j] + 1, crow[j + 1]):
+                acc[(2, jj)] += 1
+    text = gen_jac_decomp_pf(mech, 16)[0]
+    for mo in re.finditer(r"RD\((k|v), (\d+)\)", text):
+        acc[(0 if mo.group(1) == "k" else 1, int(mo.group(2)))] += 1
+    for mo in re.finditer(r"RD\((k|v), (\d+)\)", gen_fun_pf(mech, 16)):
+        if mo.group(1) == "k":
+            acc[(0, int(mo.group(2)))] += 2
+        else:
+            acc[(1, int(mo.group(2)))] += 1
+    for i in range(59):
+        acc[(1, i)] += 3
+    keys = [(0, r) for r in range(mech["nreact"])] + [(1, i) for i in range(59)] + \
+        [(2, q) for q in range(lu["nnz"])]
+    return sorted(keys, key=lambda key: -acc[key]), acc
+
+
+HOT_BUDGETS = (160, 312)
+
+
+def gen_hot_tables(mech):
+    """Per LDS budget S (doubles per lane): slot of each rate constant, state
+    and LU entry in the lane's LDS area, -1 for entries left in HBM."""
+    ranked, _ = hot_ranking(mech)
+    sizes = (mech["nreact"], 59, mech["lu"]["nnz"])
+    out = ["// LDS-resident hot set: slot per entry (-1 = HBM workspace), per budget"]
+    for S in HOT_BUDGETS:
+        slot = [[-1] * n for n in sizes]
+        for s_, (a, i) in enumerate(ranked[:S]):
+            slot[a][i] = s_
+        for a, name in enumerate(("K", "Y", "G")):
+            out.append("__device__ constexpr int16_t HOT%d_%s[%d] = {%s};"
+                       % (S, name, sizes[a], ",".join(map(str, slot[a]))))
+    out.append("template <int S>")
+    out.append("__device__ __forceinline__ constexpr int hot_slot(int arr, int i) {")
+    for S in HOT_BUDGETS:
+        out.append("  if constexpr (S == %d) return arr == 0 ? HOT%d_K[i] : arr == 1 ? HOT%d_Y[i] : HOT%d_G[i];"
+                   % (S, S, S, S))
+    out.append("  return -1;")
+    out.append("}")
+    return "\n".join(out)
+
+
 def generate(dst=None, mech_path=None):
     mech = load_mech(mech_path)
     dst = dst or os.path.join(PKG, "csrc", "generated", "radm2_gen.h")
@@ -646,6 +709,8 @@ def generate(dst=None, mech_path=None):
         gen_jac_decomp_pf(mech, 16)[0],
         "",
         gen_fun_pf(mech, 16),
+        "",
+        gen_hot_tables(mech),
         "",
         "}  // namespace gen",
         "}  // namespace radm2",

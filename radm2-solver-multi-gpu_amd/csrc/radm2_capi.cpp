@@ -73,11 +73,12 @@ constexpr int kDefaultGridBlock = 64;
 constexpr int kDefaultPersistentBlock = 64;
 // Defaults measured on MI355X at 1M cells (the autotuner may override):
 // persistent schedule, explicit workspace placement with its load-scheduled
-// kernels at 1 wave per SIMD (all 512 registers of a lane), refill as soon
-// as 4 lanes of a wave are idle.
+// kernels and the 160 most accessed doubles per lane in LDS (two 64-lane
+// blocks per CU, 1 wave per SIMD register budget), refill as soon as 4
+// lanes of a wave are idle.
 constexpr int kDefaultRefillMin = 4;
 constexpr int kDefaultWaves = 1;
-constexpr int kDefaultPlacement = radm2::PLACE_WS;
+constexpr int kDefaultPlacement = radm2::PLACE_HOT160;
 
 const radm2::KernelVariant* pick_variant(const radm2_ctx* c) {
   const int w = c->waves ? c->waves : kDefaultWaves;
@@ -208,6 +209,7 @@ int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
   const int block = pick_block(c);
   const radm2::KernelVariant* v = pick_variant(c);
   if (!v) return RADM2_E_ARG;
+  if (v->placement >= radm2::PLACE_HOT160 && block != 64) return RADM2_E_ARG;
   int64_t blocks = 0, waves = 0;
   if (c->schedule == RADM2_SCHED_GRID) {
     // every wave of the launched grid owns a slot, padding waves included
@@ -220,7 +222,7 @@ int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
     blocks = std::max<int64_t>(1, std::min(blocks, need));
     waves = blocks * (block / radm2::WAVE);
   }
-  if (v->placement == radm2::PLACE_WS) {
+  if (v->placement >= radm2::PLACE_WS) {
     if (grow) {
       if (int rc = ensure_ws(st, ws, ws_bytes, waves)) return rc;
     } else if (*ws_bytes < (size_t)waves * radm2::WS_DOUBLES_PER_LANE * radm2::WAVE * sizeof(double)) {
@@ -323,7 +325,7 @@ int radm2_set_schedule(radm2_ctx* c, int schedule, int block, int refill_min) {
 
 int radm2_set_kernel(radm2_ctx* c, int placement, int waves_per_simd) {
   if (!c) return RADM2_E_ARG;
-  if (placement < -1 || placement > 1) return RADM2_E_ARG;
+  if (placement < -1 || placement > radm2::PLACE_HOT312) return RADM2_E_ARG;
   if (waves_per_simd != 0 && waves_per_simd != 1 && waves_per_simd != 2 && waves_per_simd != 4 &&
       waves_per_simd != 8)
     return RADM2_E_ARG;
@@ -422,7 +424,7 @@ int radm2_plan_create(radm2_ctx* c, int ndom, const radm2_domain* doms, radm2_pl
       const int per_cu = v->blocks_per_cu(block);
       waves = std::min<int64_t>(waves, (int64_t)std::max(per_cu, 1) * c->cu_count * (block / radm2::WAVE));
     }
-    if (v && v->placement == radm2::PLACE_WS && ensure_ws(cap, &p->d_ws, &p->ws_bytes, waves) != 0) {
+    if (v && v->placement >= radm2::PLACE_WS && ensure_ws(cap, &p->d_ws, &p->ws_bytes, waves) != 0) {
       hipStreamDestroy(cap);
       return fail(last_hip_error());
     }

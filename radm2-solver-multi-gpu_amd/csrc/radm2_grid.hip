@@ -3,6 +3,12 @@
 // mgpu.cu:3504-3641, with the per-cell loop moved onto the lane).
 #include "radm2_cell_io.h"
 
+// placement tests below compare against radm2.h's macros (an unknown name
+// in #if would silently read as 0)
+#if !defined(RADM2_PLACE_PRIVATE) || !defined(RADM2_PLACE_LDS160) || !defined(RADM2_PLACE_LDS312)
+#error "radm2.h placement macros missing"
+#endif
+
 namespace radm2 {
 
 #ifndef RADM2_WAVES
@@ -40,17 +46,29 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
   const int di = find_domain(a.doms, a.ndom, gc);
   const DomainDesc& d = a.doms[di];
   const int64_t p = gc - d.first;
-#if RADM2_PLACE == PLACE_PRIVATE
+#if RADM2_PLACE == RADM2_PLACE_PRIVATE
   double kk[NREACT], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
 #else
   const int64_t wslot = __builtin_amdgcn_readfirstlane((int)(g / WAVE));
   constexpr int kWsBytes = WS_DOUBLES_PER_LANE * WAVE * 8;
   double* wbase = a.ws + wslot * ((int64_t)WS_DOUBLES_PER_LANE * WAVE);
-  const int lane = threadIdx.x & (WAVE - 1);
-  WsView kk = make_view(wbase, WS_K, lane, kWsBytes), y = make_view(wbase, WS_Y, lane, kWsBytes);
-  WsView lu = make_view(wbase, WS_LU, lane, kWsBytes), f = make_view(wbase, WS_F, lane, kWsBytes);
-  WsView k1 = make_view(wbase, WS_K1, lane, kWsBytes), k2 = make_view(wbase, WS_K2, lane, kWsBytes);
-  WsView k3 = make_view(wbase, WS_K3, lane, kWsBytes);
+  const int wlane = threadIdx.x & (WAVE - 1);
+  WsView f = make_view(wbase, WS_F, wlane, kWsBytes);
+  WsView k1 = make_view(wbase, WS_K1, wlane, kWsBytes);
+  WsView k2 = make_view(wbase, WS_K2, wlane, kWsBytes);
+  WsView k3 = make_view(wbase, WS_K3, wlane, kWsBytes);
+#if RADM2_PLACE >= RADM2_PLACE_LDS160
+  // the hot set's LDS slots: kHotSlots doubles per lane
+  constexpr int kHotSlots = hot_slots(RADM2_PLACE);
+  __shared__ double hot_lds[kHotSlots * BLOCK];
+  lds_double* wl = (lds_double*)hot_lds + (threadIdx.x / WAVE) * (kHotSlots * WAVE);
+  auto kk = make_hot<0, kHotSlots>(make_view(wbase, WS_K, wlane, kWsBytes), wl, wlane);
+  auto y = make_hot<1, kHotSlots>(make_view(wbase, WS_Y, wlane, kWsBytes), wl, wlane);
+  auto lu = make_hot<2, kHotSlots>(make_view(wbase, WS_LU, wlane, kWsBytes), wl, wlane);
+#else
+  WsView kk = make_view(wbase, WS_K, wlane, kWsBytes), y = make_view(wbase, WS_Y, wlane, kWsBytes);
+  WsView lu = make_view(wbase, WS_LU, wlane, kWsBytes);
+#endif
 #endif
   double h2o, m;
   CellState s;
@@ -80,9 +98,15 @@ hipError_t RADM2_NAME(launch_grid)(int block, const LaunchArgs& a, hipStream_t s
   const int64_t nb = (a.total + block - 1) / block;
   switch (block) {
     case 64: hipLaunchKernelGGL((k_ros3_grid<64, false>), dim3((unsigned)nb), dim3(64), 0, st, a); break;
+#if RADM2_PLACE < RADM2_PLACE_LDS160
     case 256: hipLaunchKernelGGL((k_ros3_grid<256, true>), dim3((unsigned)nb), dim3(256), 0, st, a); break;
+#endif
+#if RADM2_PLACE < RADM2_PLACE_LDS160
     case 512: hipLaunchKernelGGL((k_ros3_grid<512, true>), dim3((unsigned)nb), dim3(512), 0, st, a); break;
+#endif
+#if RADM2_PLACE < RADM2_PLACE_LDS160
     case 1024: hipLaunchKernelGGL((k_ros3_grid<1024, true>), dim3((unsigned)nb), dim3(1024), 0, st, a); break;
+#endif
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "radm2.h"
 #include "radm2_solver.h"
 
 namespace radm2 {
@@ -54,8 +55,14 @@ constexpr int WS_K3 = WS_K2 + NVAR;
 constexpr int WS_DOUBLES_PER_LANE = WS_K3 + NVAR;
 
 // Placement of the lane-private working arrays
-constexpr int PLACE_PRIVATE = 0;  // compiler-managed (VGPR/AGPR, scratch spills)
-constexpr int PLACE_WS = 1;       // explicit wave-interleaved HBM workspace
+// (preprocessor tests in the kernels use the RADM2_PLACE_* macros of radm2.h)
+constexpr int PLACE_PRIVATE = RADM2_PLACE_PRIVATE;  // compiler-managed (VGPR/AGPR, scratch spills)
+constexpr int PLACE_WS = RADM2_PLACE_WORKSPACE;     // explicit wave-interleaved HBM workspace
+// workspace with the most accessed entries in LDS (64-lane blocks only):
+// 160 doubles per lane (two blocks per CU) or 312 (one block per CU)
+constexpr int PLACE_HOT160 = RADM2_PLACE_LDS160;
+constexpr int PLACE_HOT312 = RADM2_PLACE_LDS312;
+constexpr int hot_slots(int placement) { return placement == PLACE_HOT160 ? 160 : 312; }
 
 // Kernel variants: one translation unit per (placement P, register budget
 // W = minimum resident waves per SIMD (1, 2, 4, 8), launder interval R of the
@@ -73,6 +80,8 @@ RADM2_DECLARE_VARIANT(0, 8, 0)
 RADM2_DECLARE_VARIANT(1, 1, 0)
 RADM2_DECLARE_VARIANT(1, 2, 0)
 RADM2_DECLARE_VARIANT(1, 4, 0)
+RADM2_DECLARE_VARIANT(2, 1, 0)
+RADM2_DECLARE_VARIANT(3, 1, 0)
 #undef RADM2_DECLARE_VARIANT
 
 struct KernelVariant {
@@ -83,7 +92,7 @@ struct KernelVariant {
   hipError_t (*persistent)(int, int, const LaunchArgs&, hipStream_t);
   int (*blocks_per_cu)(int);
 };
-constexpr int kNumVariants = 7;
+constexpr int kNumVariants = 9;
 const KernelVariant* kernel_variants();
 
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);

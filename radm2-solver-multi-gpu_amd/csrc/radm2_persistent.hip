@@ -8,6 +8,12 @@
 // whole wave hostage to its slowest lane.
 #include "radm2_cell_io.h"
 
+// placement tests below compare against radm2.h's macros (an unknown name
+// in #if would silently read as 0)
+#if !defined(RADM2_PLACE_PRIVATE) || !defined(RADM2_PLACE_LDS160) || !defined(RADM2_PLACE_LDS312)
+#error "radm2.h placement macros missing"
+#endif
+
 namespace radm2 {
 
 #ifndef RADM2_WAVES
@@ -33,18 +39,29 @@ namespace RADM2_VARIANT_NS {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_persistent(LaunchArgs a) {
-#if RADM2_PLACE == PLACE_PRIVATE
+#if RADM2_PLACE == RADM2_PLACE_PRIVATE
   double kk[NREACT], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
 #else
   const int64_t wslot = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE));
   constexpr int kWsBytes = WS_DOUBLES_PER_LANE * WAVE * 8;
   double* wbase = a.ws + wslot * ((int64_t)WS_DOUBLES_PER_LANE * WAVE);
-  WsView kk = make_view(wbase, WS_K, threadIdx.x & (WAVE - 1), kWsBytes), y = make_view(wbase, WS_Y, threadIdx.x & (WAVE - 1), kWsBytes);
-  WsView lu = make_view(wbase, WS_LU, threadIdx.x & (WAVE - 1), kWsBytes);
-  WsView f = make_view(wbase, WS_F, threadIdx.x & (WAVE - 1), kWsBytes);
-  WsView k1 = make_view(wbase, WS_K1, threadIdx.x & (WAVE - 1), kWsBytes);
-  WsView k2 = make_view(wbase, WS_K2, threadIdx.x & (WAVE - 1), kWsBytes);
-  WsView k3 = make_view(wbase, WS_K3, threadIdx.x & (WAVE - 1), kWsBytes);
+  const int wlane = threadIdx.x & (WAVE - 1);
+  WsView f = make_view(wbase, WS_F, wlane, kWsBytes);
+  WsView k1 = make_view(wbase, WS_K1, wlane, kWsBytes);
+  WsView k2 = make_view(wbase, WS_K2, wlane, kWsBytes);
+  WsView k3 = make_view(wbase, WS_K3, wlane, kWsBytes);
+#if RADM2_PLACE >= RADM2_PLACE_LDS160
+  // the hot set's LDS slots: kHotSlots doubles per lane
+  constexpr int kHotSlots = hot_slots(RADM2_PLACE);
+  __shared__ double hot_lds[kHotSlots * BLOCK];
+  lds_double* wl = (lds_double*)hot_lds + (threadIdx.x / WAVE) * (kHotSlots * WAVE);
+  auto kk = make_hot<0, kHotSlots>(make_view(wbase, WS_K, wlane, kWsBytes), wl, wlane);
+  auto y = make_hot<1, kHotSlots>(make_view(wbase, WS_Y, wlane, kWsBytes), wl, wlane);
+  auto lu = make_hot<2, kHotSlots>(make_view(wbase, WS_LU, wlane, kWsBytes), wl, wlane);
+#else
+  WsView kk = make_view(wbase, WS_K, wlane, kWsBytes), y = make_view(wbase, WS_Y, wlane, kWsBytes);
+  WsView lu = make_view(wbase, WS_LU, wlane, kWsBytes);
+#endif
 #endif
   double h2o = 0.0, m = 0.0;
   CellState s;
@@ -112,7 +129,9 @@ hipError_t RADM2_NAME(launch_persistent)(int block, int blocks, const LaunchArgs
   if (a.total <= 0) return hipSuccess;
   switch (block) {
     case 64: return launch_p<64>(blocks, a, st);
+#if RADM2_PLACE < RADM2_PLACE_LDS160
     case 256: return launch_p<256>(blocks, a, st);
+#endif
     default: return hipErrorInvalidValue;
   }
 }
@@ -122,7 +141,9 @@ int RADM2_NAME(persistent_blocks_per_cu)(int block) {
   hipError_t e = hipErrorInvalidValue;
   switch (block) {
     case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_persistent<64>, 64, 0); break;
+#if RADM2_PLACE < RADM2_PLACE_LDS160
     case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_persistent<256>, 256, 0); break;
+#endif
     default: break;
   }
   return e == hipSuccess ? nb : -1;

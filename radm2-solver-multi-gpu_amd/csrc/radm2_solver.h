@@ -21,6 +21,7 @@
 
 #include "radm2_ws.h"
 #include "generated/radm2_gen.h"
+#include "radm2_hot.h"
 
 namespace radm2 {
 
@@ -122,47 +123,51 @@ __device__ __forceinline__ void cell_attempt_top(CellState& s, const Ros3Opts& o
 // schedule.  Both compute the same operations in the same order.
 template <class GA, class XA>
 __device__ __forceinline__ void stage_solve(GA g, XA x) {
-  if constexpr (__is_same(GA, WsView)) gen::solve_pf16(g, x);
+  if constexpr (is_ws_like<GA>::value) gen::solve_pf16(g, x);
   else gen::solve(g, x);
 }
 
 template <int R, class KA, class YA, class GA>
 __device__ __forceinline__ int stage_jac_decomp(KA k, YA y, double h2o, double m, double ghinv,
                                                 GA g) {
-  if constexpr (__is_same(GA, WsView)) return gen::jac_decomp_pf16(k, y, h2o, m, ghinv, g);
+  if constexpr (is_ws_like<GA>::value) return gen::jac_decomp_pf16(k, y, h2o, m, ghinv, g);
   else return gen::jac_decomp<R>(k, y, h2o, m, ghinv, g);
 }
 
 template <int RF, class KA, class VA, class FA>
 __device__ __forceinline__ void stage_fun(KA k, VA v, double h2o, double m, FA f) {
-  if constexpr (__is_same(FA, WsView)) gen::fun_pf16(k, v, h2o, m, f);
+  if constexpr (is_ws_like<FA>::value) gen::fun_pf16(k, v, h2o, m, f);
   else gen::fun<RF>(k, v, h2o, m, f);
 }
 
-// Element-wise pass out[i] = fn(i, {in[0][i], .., in[N-1][i]}) over the 59
+// Element-wise pass out[i] = fn(i, {in0[i], in1[i], ..}) over the 59
 // entries, CH at a time: all inputs of a chunk are loaded before any of its
 // outputs is stored, and the next chunk's inputs before this chunk's stores
 // (an output never aliases an input), so on workspace views a pass costs
 // about 59/CH memory round trips instead of one per entry.  fn runs for
 // i = 0, 1, .., 58 in that order.
-template <int CH, int N, class OA, class IA, class Fn>
-__device__ __forceinline__ void vmap(OA out, const IA (&in)[N], Fn fn) {
+template <int CH, class A>
+__device__ __forceinline__ void vload(double (&dst)[CH], const A& a, int c) {
+#pragma unroll
+  for (int j = 0; j < CH; j++)
+    if (c * CH + j < NVAR) dst[j] = RD(a, c * CH + j);
+}
+
+template <int CH, class OA, class Fn, class... IA>
+__device__ __forceinline__ void vmap(OA out, Fn fn, const IA&... in) {
+  constexpr int N = sizeof...(IA);
   constexpr int NCH = (NVAR + CH - 1) / CH;
   double buf[2][N][CH];
-#pragma unroll
-  for (int t = 0; t < N; t++)
-#pragma unroll
-    for (int j = 0; j < CH; j++)
-      if (j < NVAR) buf[0][t][j] = RD(in[t], j);
+  {
+    int t = 0;
+    (vload<CH>(buf[0][t++], in, 0), ...);
+  }
 #pragma unroll
   for (int c = 0; c < NCH; c++) {
     const int cur = c & 1;
     if (c + 1 < NCH) {
-#pragma unroll
-      for (int t = 0; t < N; t++)
-#pragma unroll
-        for (int j = 0; j < CH; j++)
-          if ((c + 1) * CH + j < NVAR) buf[cur ^ 1][t][j] = RD(in[t], (c + 1) * CH + j);
+      int t = 0;
+      (vload<CH>(buf[cur ^ 1][t++], in, c + 1), ...);
     }
 #pragma unroll
     for (int j = 0; j < CH; j++) {
@@ -211,31 +216,22 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
         // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
         launder(k1);
         launder(y);
-        {
-          const VA in[2] = {y, k1};
-          vmap<16>(k2, in, [](int, const double* a) { return a[0] + a[1]; });
-        }
+        vmap<16>(k2, [](int, const double* a) { return a[0] + a[1]; }, y, k1);
         launder(k2);
         launder(k);
         stage_fun<RF>(k, k2, h2o, m, f);
         launder(f);
         launder(k1);
-        {
-          const VA in[2] = {k1, f};
-          vmap<16>(k2, in, [hc21](int, const double* a) { return fma(a[0], hc21, a[1]); });
-        }
+        vmap<16>(k2, [hc21](int, const double* a) { return fma(a[0], hc21, a[1]); }, k1, f);
         region = k2.region;
       } else if (st == 2) {
         // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
         launder(k2);
         launder(f);
         launder(k1);
-        {
-          const VA in[3] = {k2, k1, f};
-          vmap<16>(k3, in, [hc31, hc32](int, const double* a) {
-            return fma(a[0], hc32, fma(a[1], hc31, a[2]));
-          });
-        }
+        vmap<16>(k3, [hc31, hc32](int, const double* a) {
+          return fma(a[0], hc32, fma(a[1], hc31, a[2]));
+        }, k2, k1, f);
         region = k3.region;
       }
       WsView x = k1;
@@ -247,23 +243,14 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
   } else {
     stage_solve(g, k1);
     // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
-    {
-      const VA in[2] = {y, k1};
-      vmap<16>(k2, in, [](int, const double* a) { return a[0] + a[1]; });
-    }
+    vmap<16>(k2, [](int, const double* a) { return a[0] + a[1]; }, y, k1);
     stage_fun<RF>(k, k2, h2o, m, f);
-    {
-      const VA in[2] = {k1, f};
-      vmap<16>(k2, in, [hc21](int, const double* a) { return fma(a[0], hc21, a[1]); });
-    }
+    vmap<16>(k2, [hc21](int, const double* a) { return fma(a[0], hc21, a[1]); }, k1, f);
     stage_solve(g, k2);
     // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
-    {
-      const VA in[3] = {k2, k1, f};
-      vmap<16>(k3, in, [hc31, hc32](int, const double* a) {
-        return fma(a[0], hc32, fma(a[1], hc31, a[2]));
-      });
-    }
+    vmap<16>(k3, [hc31, hc32](int, const double* a) {
+      return fma(a[0], hc32, fma(a[1], hc31, a[2]));
+    }, k2, k1, f);
     stage_solve(g, k3);
   }
   launder(k1);
@@ -273,9 +260,8 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
   // combination and WRMS error norm; Ynew kept in f until the decision
   double err = 0.0;
   {
-    const VA in[4] = {k1, k2, k3, y};
     const double rtol = o.rtol, atol = o.atol;
-    vmap<8>(f, in, [&err, rtol, atol](int, const double* a) {
+    vmap<8>(f, [&err, rtol, atol](int, const double* a) {
       const double a1 = a[0], a2 = a[1], a3 = a[2], yi = a[3];
       const double sm = fma(ROS_M3, a3, fma(ROS_M2, a2, ROS_M1 * a1));
       const double yn = yi + sm;
@@ -284,7 +270,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
       const double q = e / fma(rtol, ymax, atol);
       err = fma(q, q, err);
       return yn;
-    });
+    }, k1, k2, k3, y);
   }
   err = sqrt(err / (double)NVAR);
   const double fsafe = dmax_ref(o.facmin, o.facsafe / pow(err, 1.0 / ROS_ELO));
@@ -294,10 +280,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
   if (err <= 1.0 || H <= o.hmin) {
     launder(f);
     launder(y);
-    {
-      const VA in[1] = {f};
-      vmap<16>(y, in, [](int, const double* a) { return a[0]; });
-    }
+    vmap<16>(y, [](int, const double* a) { return a[0]; }, f);
     s.T = s.T + H;
     if (s.na == 0) s.hfirst = H * fsafe;  // uncapped proposal (learned-h0 target)
     if (s.rejLast) Hnew = dmin_ref(Hnew, H);

@@ -34,6 +34,8 @@ SCHED_GRID = 0
 SCHED_PERSISTENT = 1
 PLACE_PRIVATE = 0
 PLACE_WORKSPACE = 1
+PLACE_LDS160 = 2
+PLACE_LDS312 = 3
 
 # reference species order (mgpu.cu:125-183)
 SPECIES = (
@@ -78,7 +80,7 @@ _EXPORTS = [
     ("radm2_set_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("radm2_set_kernel", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     ("radm2_integrate", ctypes.c_int,
-     [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_void_p] * 7
+     [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_void_p] * 6
      + [ctypes.c_double] * 3 + [ctypes.POINTER(Opts)] + [ctypes.c_void_p] * 6),
     ("radm2_integrate_batched", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Domain)]),
     ("radm2_plan_create", ctypes.c_int,
@@ -253,6 +255,19 @@ class Solver:
         d = batch.domain(dt, rtol, atol, opts, ncells)
         self._before()
         code = self.lib.radm2_integrate_batched(self.ctx, 1, ctypes.byref(d))
+        self._after()
+        _check(code)
+
+    def integrate_single(self, batch, dt, rtol=1e-3, atol=1.0, opts=None):
+        """The single-domain entry point radm2_integrate (same semantics as
+        integrate(), which goes through radm2_integrate_batched)."""
+        self._before()
+        code = self.lib.radm2_integrate(
+            self.ctx, batch.ncells, batch.ld, _ptr(batch.conc), _ptr(batch.temp),
+            _ptr(batch.cair), _ptr(batch.h2o), _ptr(batch.jrate), _ptr(batch.rc_n2o5),
+            float(dt), float(rtol), float(atol), ctypes.pointer(opts) if opts is not None else None,
+            _ptr(batch.h0), _ptr(batch.nstep), _ptr(batch.nacc), _ptr(batch.nrej),
+            _ptr(batch.status), _ptr(batch.hexit))
         self._after()
         _check(code)
 

@@ -56,9 +56,11 @@ def test_all_variants_and_schedules_bit_identical(solver, cells):
            if k in ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")}
     ref = None
     configs = []
-    for pl, w in ((0, 1), (0, 2), (0, 4), (0, 8), (1, 1), (1, 2), (1, 4)):
+    for pl, w in ((0, 1), (0, 2), (0, 4), (0, 8), (1, 1), (1, 2), (1, 4), (2, 1), (3, 1)):
         for sched, block in ((radm2_amd.SCHED_GRID, 64), (radm2_amd.SCHED_GRID, 256),
                              (radm2_amd.SCHED_PERSISTENT, 64), (radm2_amd.SCHED_PERSISTENT, 256)):
+            if pl >= radm2_amd.PLACE_LDS160 and block != 64:
+                continue          # LDS placements run 64-lane blocks only
             configs.append((pl, w, sched, block))
     for pl, w, sched, block in configs:
         b = _batch(sub)
@@ -74,6 +76,36 @@ def test_all_variants_and_schedules_bit_identical(solver, cells):
             assert _close(out["conc"], co).mean() >= 0.99
         for k in ref:
             np.testing.assert_array_equal(out[k], ref[k], err_msg=str((pl, w, sched, block, k)))
+    solver.set_schedule(1, 0, 0)
+    solver.set_kernel(-1, 0)
+
+
+def test_single_domain_entry_point_matches_batched(solver, cells):
+    # radm2_integrate (explicit arguments) and radm2_integrate_batched (a
+    # domain table) run the same kernels: bit-identical results
+    sub = {k: np.ascontiguousarray(v[..., :300]) for k, v in cells.items()
+           if k in ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")}
+    a, b = _batch(sub), _batch(sub)
+    solver.integrate(a, 600.0, 1e-3, 1.0)
+    solver.integrate_single(b, 600.0, 1e-3, 1.0)
+    solver.sync()
+    oa, ob = _out(a), _out(b)
+    for k in oa:
+        np.testing.assert_array_equal(oa[k], ob[k], err_msg=k)
+    assert (ob["status"] == 1).all()
+
+
+def test_lds_placements_reject_wide_blocks(solver, cells):
+    import radm2_amd
+    sub = {k: np.ascontiguousarray(v[..., :64]) for k, v in cells.items()
+           if k in ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")}
+    b = _batch(sub)
+    for pl in (radm2_amd.PLACE_LDS160, radm2_amd.PLACE_LDS312):
+        solver.set_kernel(pl, 1)
+        solver.set_schedule(radm2_amd.SCHED_PERSISTENT, 256, 0)
+        with pytest.raises(radm2_amd.Radm2Error) as e:
+            solver.integrate(b, 600.0, 1e-3, 1.0)
+        assert e.value.code == radm2_amd.E_ARG
     solver.set_schedule(1, 0, 0)
     solver.set_kernel(-1, 0)
 
