@@ -108,8 +108,10 @@ int radm2_set_schedule(radm2_ctx *ctx, int schedule, int block, int refill_min);
  * (compiler-managed registers, spills to scratch), RADM2_PLACE_WORKSPACE
  * (an explicit wave-interleaved HBM workspace whose loads are issued in
  * batches ahead of their use), RADM2_PLACE_LDS160 / RADM2_PLACE_LDS312 (the
- * workspace with its 160 / 312 most accessed doubles per lane held in LDS;
- * 64-lane blocks only, other block sizes give RADM2_E_ARG at launch);
+ * workspace with its 160 / 312 most accessed doubles per lane held in LDS),
+ * RADM2_PLACE_LDS160_REG128 (160 in LDS and the next 128 in registers; the
+ * default) -- the LDS placements run 64-lane blocks only, other block sizes
+ * give RADM2_E_ARG at launch;
  * -1 = tuned default -- and the register budget as the minimum resident
  * waves per SIMD (1, 2, 4 or 8; 0 = tuned default).  Returns
  * RADM2_E_UNSUPPORTED for a combination that is not compiled in.  No
@@ -118,6 +120,7 @@ int radm2_set_schedule(radm2_ctx *ctx, int schedule, int block, int refill_min);
 #define RADM2_PLACE_WORKSPACE 1
 #define RADM2_PLACE_LDS160 2
 #define RADM2_PLACE_LDS312 3
+#define RADM2_PLACE_LDS160_REG128 4
 int radm2_set_kernel(radm2_ctx *ctx, int placement, int waves_per_simd);
 
 /*

@@ -624,27 +624,34 @@ def hot_ranking(mech):
     return sorted(keys, key=lambda key: -acc[key]), acc
 
 
-HOT_BUDGETS = (160, 312)
+# (LDS slots, register entries) per lane of each hot-set configuration
+HOT_BUDGETS = ((160, 0), (312, 0), (160, 128))
 
 
 def gen_hot_tables(mech):
-    """Per LDS budget S (doubles per lane): slot of each rate constant, state
-    and LU entry in the lane's LDS area, -1 for entries left in HBM."""
+    """Per hot-set configuration (S LDS slots, R registers per lane): the
+    code of each rate constant, state and LU entry -- s >= 0 for LDS slot s
+    (the first S of the ranking), -2 - r for register r (the next R), -1
+    for the HBM workspace."""
     ranked, _ = hot_ranking(mech)
     sizes = (mech["nreact"], 59, mech["lu"]["nnz"])
-    out = ["// LDS-resident hot set: slot per entry (-1 = HBM workspace), per budget"]
-    for S in HOT_BUDGETS:
-        slot = [[-1] * n for n in sizes]
+    out = ["// hot set per (LDS slots, registers): code per entry (s >= 0 LDS slot s,"
+           " -2 - r register r, -1 HBM)"]
+    for S, R in HOT_BUDGETS:
+        code = [[-1] * n for n in sizes]
         for s_, (a, i) in enumerate(ranked[:S]):
-            slot[a][i] = s_
+            code[a][i] = s_
+        for r, (a, i) in enumerate(ranked[S:S + R]):
+            code[a][i] = -2 - r
         for a, name in enumerate(("K", "Y", "G")):
-            out.append("__device__ constexpr int16_t HOT%d_%s[%d] = {%s};"
-                       % (S, name, sizes[a], ",".join(map(str, slot[a]))))
-    out.append("template <int S>")
-    out.append("__device__ __forceinline__ constexpr int hot_slot(int arr, int i) {")
-    for S in HOT_BUDGETS:
-        out.append("  if constexpr (S == %d) return arr == 0 ? HOT%d_K[i] : arr == 1 ? HOT%d_Y[i] : HOT%d_G[i];"
-                   % (S, S, S, S))
+            out.append("__device__ constexpr int16_t HOT%d_%d_%s[%d] = {%s};"
+                       % (S, R, name, sizes[a], ",".join(map(str, code[a]))))
+    out.append("template <int S, int R>")
+    out.append("__device__ __forceinline__ constexpr int hot_code(int arr, int i) {")
+    for S, R in HOT_BUDGETS:
+        out.append("  if constexpr (S == %d && R == %d)" % (S, R))
+        out.append("    return arr == 0 ? HOT%d_%d_K[i] : arr == 1 ? HOT%d_%d_Y[i] : HOT%d_%d_G[i];"
+                   % (S, R, S, R, S, R))
     out.append("  return -1;")
     out.append("}")
     return "\n".join(out)

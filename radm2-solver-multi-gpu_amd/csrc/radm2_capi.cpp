@@ -73,12 +73,12 @@ constexpr int kDefaultGridBlock = 64;
 constexpr int kDefaultPersistentBlock = 64;
 // Defaults measured on MI355X at 1M cells (the autotuner may override):
 // persistent schedule, explicit workspace placement with its load-scheduled
-// kernels and the 160 most accessed doubles per lane in LDS (two 64-lane
-// blocks per CU, 1 wave per SIMD register budget), refill as soon as 4
-// lanes of a wave are idle.
+// kernels, the 160 most accessed doubles per lane in LDS (two 64-lane
+// blocks per CU) and the next 128 in registers (1 wave per SIMD register
+// budget), refill as soon as 4 lanes of a wave are idle.
 constexpr int kDefaultRefillMin = 4;
 constexpr int kDefaultWaves = 1;
-constexpr int kDefaultPlacement = radm2::PLACE_HOT160;
+constexpr int kDefaultPlacement = radm2::PLACE_HOT160R128;
 
 const radm2::KernelVariant* pick_variant(const radm2_ctx* c) {
   const int w = c->waves ? c->waves : kDefaultWaves;
@@ -325,7 +325,7 @@ int radm2_set_schedule(radm2_ctx* c, int schedule, int block, int refill_min) {
 
 int radm2_set_kernel(radm2_ctx* c, int placement, int waves_per_simd) {
   if (!c) return RADM2_E_ARG;
-  if (placement < -1 || placement > radm2::PLACE_HOT312) return RADM2_E_ARG;
+  if (placement < -1 || placement > radm2::PLACE_HOT_LAST) return RADM2_E_ARG;
   if (waves_per_simd != 0 && waves_per_simd != 1 && waves_per_simd != 2 && waves_per_simd != 4 &&
       waves_per_simd != 8)
     return RADM2_E_ARG;

@@ -58,13 +58,14 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
   WsView k2 = make_view(wbase, WS_K2, wlane, kWsBytes);
   WsView k3 = make_view(wbase, WS_K3, wlane, kWsBytes);
 #if RADM2_PLACE >= RADM2_PLACE_LDS160
-  // the hot set's LDS slots: kHotSlots doubles per lane
-  constexpr int kHotSlots = hot_slots(RADM2_PLACE);
+  // the hot set: kHotSlots doubles per lane in LDS, kHotRegs in registers
+  constexpr int kHotSlots = hot_slots(RADM2_PLACE), kHotRegs = hot_regs(RADM2_PLACE);
   __shared__ double hot_lds[kHotSlots * BLOCK];
+  double hot_rg[kHotRegs > 0 ? kHotRegs : 1];
   lds_double* wl = (lds_double*)hot_lds + (threadIdx.x / WAVE) * (kHotSlots * WAVE);
-  auto kk = make_hot<0, kHotSlots>(make_view(wbase, WS_K, wlane, kWsBytes), wl, wlane);
-  auto y = make_hot<1, kHotSlots>(make_view(wbase, WS_Y, wlane, kWsBytes), wl, wlane);
-  auto lu = make_hot<2, kHotSlots>(make_view(wbase, WS_LU, wlane, kWsBytes), wl, wlane);
+  auto kk = make_hot<0, kHotSlots, kHotRegs>(make_view(wbase, WS_K, wlane, kWsBytes), wl, wlane, hot_rg);
+  auto y = make_hot<1, kHotSlots, kHotRegs>(make_view(wbase, WS_Y, wlane, kWsBytes), wl, wlane, hot_rg);
+  auto lu = make_hot<2, kHotSlots, kHotRegs>(make_view(wbase, WS_LU, wlane, kWsBytes), wl, wlane, hot_rg);
 #else
   WsView kk = make_view(wbase, WS_K, wlane, kWsBytes), y = make_view(wbase, WS_Y, wlane, kWsBytes);
   WsView lu = make_view(wbase, WS_LU, wlane, kWsBytes);

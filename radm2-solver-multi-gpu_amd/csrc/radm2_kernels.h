@@ -62,7 +62,15 @@ constexpr int PLACE_WS = RADM2_PLACE_WORKSPACE;     // explicit wave-interleaved
 // 160 doubles per lane (two blocks per CU) or 312 (one block per CU)
 constexpr int PLACE_HOT160 = RADM2_PLACE_LDS160;
 constexpr int PLACE_HOT312 = RADM2_PLACE_LDS312;
-constexpr int hot_slots(int placement) { return placement == PLACE_HOT160 ? 160 : 312; }
+// (LDS slots, register entries) per lane of placements 2..4; measured at 1M
+// cells: more LDS per lane (fewer resident waves) or more registers (spills)
+// or fewer LDS slots (more waves, more HBM traffic) are all slower than
+// (160, 128)
+constexpr int PLACE_HOT160R128 = RADM2_PLACE_LDS160_REG128;
+constexpr int kHotConfig[5][2] = {{0, 0}, {0, 0}, {160, 0}, {312, 0}, {160, 128}};
+constexpr int PLACE_HOT_LAST = PLACE_HOT160R128;
+constexpr int hot_slots(int placement) { return kHotConfig[placement][0]; }
+constexpr int hot_regs(int placement) { return kHotConfig[placement][1]; }
 
 // Kernel variants: one translation unit per (placement P, register budget
 // W = minimum resident waves per SIMD (1, 2, 4, 8), launder interval R of the
@@ -82,6 +90,7 @@ RADM2_DECLARE_VARIANT(1, 2, 0)
 RADM2_DECLARE_VARIANT(1, 4, 0)
 RADM2_DECLARE_VARIANT(2, 1, 0)
 RADM2_DECLARE_VARIANT(3, 1, 0)
+RADM2_DECLARE_VARIANT(4, 1, 0)
 #undef RADM2_DECLARE_VARIANT
 
 struct KernelVariant {
@@ -92,7 +101,7 @@ struct KernelVariant {
   hipError_t (*persistent)(int, int, const LaunchArgs&, hipStream_t);
   int (*blocks_per_cu)(int);
 };
-constexpr int kNumVariants = 9;
+constexpr int kNumVariants = 10;
 const KernelVariant* kernel_variants();
 
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);

@@ -22,7 +22,7 @@ const KernelVariant* kernel_variants() {
       RADM2_VARIANT_ROW(0, 1, 0), RADM2_VARIANT_ROW(0, 2, 0), RADM2_VARIANT_ROW(0, 4, 0),
       RADM2_VARIANT_ROW(0, 8, 0),
       RADM2_VARIANT_ROW(1, 1, 0), RADM2_VARIANT_ROW(1, 2, 0), RADM2_VARIANT_ROW(1, 4, 0),
-      RADM2_VARIANT_ROW(2, 1, 0), RADM2_VARIANT_ROW(3, 1, 0),
+      RADM2_VARIANT_ROW(2, 1, 0), RADM2_VARIANT_ROW(3, 1, 0), RADM2_VARIANT_ROW(4, 1, 0),
   };
 #undef RADM2_VARIANT_ROW
   return v;

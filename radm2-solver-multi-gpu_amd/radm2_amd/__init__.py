@@ -36,6 +36,7 @@ PLACE_PRIVATE = 0
 PLACE_WORKSPACE = 1
 PLACE_LDS160 = 2
 PLACE_LDS312 = 3
+PLACE_LDS160_REG128 = 4
 
 # reference species order (mgpu.cu:125-183)
 SPECIES = (

@@ -56,7 +56,7 @@ def test_all_variants_and_schedules_bit_identical(solver, cells):
            if k in ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")}
     ref = None
     configs = []
-    for pl, w in ((0, 1), (0, 2), (0, 4), (0, 8), (1, 1), (1, 2), (1, 4), (2, 1), (3, 1)):
+    for pl, w in ((0, 1), (0, 2), (0, 4), (0, 8), (1, 1), (1, 2), (1, 4), (2, 1), (3, 1), (4, 1)):
         for sched, block in ((radm2_amd.SCHED_GRID, 64), (radm2_amd.SCHED_GRID, 256),
                              (radm2_amd.SCHED_PERSISTENT, 64), (radm2_amd.SCHED_PERSISTENT, 256)):
             if pl >= radm2_amd.PLACE_LDS160 and block != 64:
@@ -100,7 +100,7 @@ def test_lds_placements_reject_wide_blocks(solver, cells):
     sub = {k: np.ascontiguousarray(v[..., :64]) for k, v in cells.items()
            if k in ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")}
     b = _batch(sub)
-    for pl in (radm2_amd.PLACE_LDS160, radm2_amd.PLACE_LDS312):
+    for pl in (radm2_amd.PLACE_LDS160, radm2_amd.PLACE_LDS312, radm2_amd.PLACE_LDS160_REG128):
         solver.set_kernel(pl, 1)
         solver.set_schedule(radm2_amd.SCHED_PERSISTENT, 256, 0)
         with pytest.raises(radm2_amd.Radm2Error) as e:
