@@ -29,7 +29,7 @@ from . import (PLACE_LDS160, PLACE_LDS160_REG128, PLACE_LDS312, PLACE_PRIVATE, P
 KERNELS = [(PLACE_PRIVATE, 1), (PLACE_PRIVATE, 2), (PLACE_PRIVATE, 4), (PLACE_PRIVATE, 8),
            (PLACE_WORKSPACE, 1), (PLACE_WORKSPACE, 2), (PLACE_WORKSPACE, 4),
            (PLACE_LDS160, 1), (PLACE_LDS312, 1), (PLACE_LDS160_REG128, 1)]
-DEFAULT = dict(schedule=SCHED_PERSISTENT, placement=PLACE_LDS160_REG128, waves=1, block=64, refill=4)
+DEFAULT = dict(schedule=SCHED_PERSISTENT, placement=PLACE_LDS160_REG128, waves=1, block=64, refill=1)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "radm2_amd", "autotune.json")
 
 

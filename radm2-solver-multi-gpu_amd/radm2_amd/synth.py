@@ -92,6 +92,24 @@ CONFIGS = {
     "c0_4k": dict(n=4096, seed=0, zmax_deg=80.0, plume_frac=0.0),
     "c1_1m": dict(n=1048576, seed=0, zmax_deg=100.0, plume_frac=0.05),
 }
+# the batched 64-domain workload: make_domains()
+
+
+def make_domains(seed=4, ndom=64, lo=1000, hi=200000, dts=(60.0, 300.0, 600.0, 3600.0),
+                 rtols=(1e-2, 1e-3, 1e-4)):
+    """The batched workload: `ndom` domains with log-uniform sizes in
+    [lo, hi] cells (2.4 million in expectation for the defaults; 3.3 million
+    for seed 4), each with
+    its own dt drawn from `dts` and rtol from `rtols` (atol 1), cell states
+    as in the 1M-cell case (zenith U(0,100), 5% plume cells).  Returns a
+    list of (cells, dt, rtol, atol)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), ndom)).astype(np.int64)
+    out = []
+    for d in range(ndom):
+        cells = make_cells(int(sizes[d]), seed * 1000 + d, 100.0, 0.05)
+        out.append((cells, float(rng.choice(dts)), float(rng.choice(rtols)), 1.0))
+    return out
 
 
 def to_device(cells, device="cuda"):

@@ -273,3 +273,42 @@ def test_kpp_frontend_statistics(solver, cells):
     IPAR2 = [0] * 20
     IPAR2[0] = 1
     assert solver.rosenbrock(b, 0.0, 240.0, [1.0] * 59, [1e-3] * 59, [0.0] * 20, IPAR2) == -2
+
+
+def test_batched_64_uneven_domains_plan_matches_oracle(solver):
+    # the batched workload: 64 domains, log-uniform 1k..200k cells (about
+    # 2.5M), per-domain dt in {60, 300, 600, 3600} s and rtol in {1e-2, 1e-3,
+    # 1e-4}, run as one captured graph; 32 cells of every domain checked
+    # against the oracle with the domain's own dt and rtol
+    import time
+    import torch
+    from radm2_amd import synth
+    doms = synth.make_domains()
+    total = sum(c["conc"].shape[1] for c, _, _, _ in doms)
+    assert 2_000_000 <= total <= 3_500_000
+    batches = [_batch(c) for c, _, _, _ in doms]
+    conc0 = [b.conc.clone() for b in batches]
+    plan = solver.plan([b.domain(dt, rtol, atol) for b, (_, dt, rtol, atol) in zip(batches, doms)])
+    for rep in range(2):
+        for b, c0 in zip(batches, conc0):
+            b.conc.copy_(c0)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        plan.launch()
+        solver.sync()
+        el = time.perf_counter() - t
+    print("64 domains, %d cells: %.3f s, %.0f cell-integrations/s" % (total, el, total / el))
+    nsub = 32
+    worst, good = 1.0, 0
+    for b, (c, dt, rtol, atol) in zip(batches, doms):
+        st = b.status.cpu().numpy()
+        assert (st == 1).all(), (dt, rtol)
+        sub = {k: np.ascontiguousarray(v[..., :nsub]) for k, v in c.items()
+               if k in ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")}
+        co, oo, code = O.integrate(sub, dt, rtol, atol)
+        assert code == 0
+        ok = _close(b.conc[:, :nsub].cpu().numpy(), co)
+        worst = min(worst, ok.mean())
+        good += int(ok.sum())
+    assert good >= 0.98 * nsub * len(doms) and worst >= 0.9, (good, worst)
+    plan.close()
