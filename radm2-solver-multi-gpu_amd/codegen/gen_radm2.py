@@ -524,6 +524,41 @@ def gen_fun_pf(mech, G, window=16):
     return "\n".join(out)
 
 
+def gen_solve_pf_reg(mech, G):
+    """solve_pf<G>() on a register-resident right-hand side: x is a
+    59-entry local array of the caller (indexed by constants only), solved
+    in place; the LU is streamed as in solve_pf<G>()."""
+    text = gen_solve_pf(mech, G)
+    lines = text.split("\n")
+    out = ["// solve_pf%d() with x held by the caller (registers), solved in place" % G,
+           "template <class GA>",
+           "__device__ __forceinline__ void solve_pf%d_reg(GA g, double (&x)[59]) {" % G]
+    import re
+    for ln in lines[3:]:
+        if re.match(r"\s*double x\d+ = RD\(x, \d+\);", ln) or re.match(r"\s*WR\(x, \d+, x\d+\);", ln):
+            continue
+        out.append(re.sub(r"\bx(\d+)\b", r"x[\1]", ln))
+    return "\n".join(out)
+
+
+def gen_fun_pf_reg(mech, G, window=16):
+    """fun_pf<G>() leaving the tendency in a 59-entry local array of the
+    caller (registers) instead of storing it."""
+    import re
+    text = gen_fun_pf(mech, G, window)
+    lines = text.split("\n")
+    out = ["// fun_pf%d() with the tendency left in the caller's registers" % G,
+           "template <class KA, class VA>",
+           "__device__ __forceinline__ void fun_pf%d_reg(KA k, VA v, double h2o, double m,"
+           " double (&f)[59]) {" % G]
+    for ln in lines[3:]:
+        if re.match(r"\s*double f\d+(, f\d+)*;", ln) or re.match(r"\s*WR\(f, \d+, f\d+\);", ln):
+            continue
+        ln = re.sub(r"\s*double f(\d+) = 0\.0;", r"  f[\1] = 0.0;", ln)
+        out.append(re.sub(r"\bf(\d+)\b", r"f[\1]", ln))
+    return "\n".join(out)
+
+
 def gen_jac_decomp_pf(mech, G):
     """jac_decomp() with every load scheduled explicitly: the same
     operations in the same order, but rate constants, concentrations and
@@ -716,6 +751,10 @@ def generate(dst=None, mech_path=None):
         gen_jac_decomp_pf(mech, 16)[0],
         "",
         gen_fun_pf(mech, 16),
+        "",
+        gen_solve_pf_reg(mech, 16),
+        "",
+        gen_fun_pf_reg(mech, 16),
         "",
         gen_hot_tables(mech),
         "",

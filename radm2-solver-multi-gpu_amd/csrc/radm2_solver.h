@@ -188,59 +188,100 @@ __device__ __forceinline__ void vmap(OA out, Fn fn, const IA&... in) {
 // R: launder interval of the compiler-scheduled fused Jacobian/decomposition
 // (rows) and, as 16 reactions per unit, of the tendency on private arrays;
 // phases are laundered apart whenever the arrays are workspace views.
+// fn(i, {in0[i], in1[i], ..}) for i = 0, 1, .., 58 in order, the inputs
+// streamed CH at a time with the next chunk's loads issued before this
+// chunk's work (fenced, so the compiler keeps them apart); fn may store to
+// views that none of the inputs alias.
+template <int CH, class Fn, class... IA>
+__device__ __forceinline__ void xpass(Fn fn, const IA&... in) {
+  constexpr int N = sizeof...(IA);
+  constexpr int NCH = (NVAR + CH - 1) / CH;
+  double buf[2][N][CH];
+  {
+    int t = 0;
+    (vload<CH>(buf[0][t++], in, 0), ...);
+  }
+#pragma unroll
+  for (int c = 0; c < NCH; c++) {
+    const int cur = c & 1;
+    if (c + 1 < NCH) {
+      int t = 0;
+      (vload<CH>(buf[cur ^ 1][t++], in, c + 1), ...);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+      const int i = c * CH + j;
+      if (i < NVAR) {
+        double v[N];
+#pragma unroll
+        for (int t = 0; t < N; t++) v[t] = buf[cur][t][j];
+        fn(i, v);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 template <int R, class KA, class YA, class GA, class VA>
 __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA k, YA y,
                                              double h2o, double m, GA g, VA f, VA k1, VA k2,
                                              VA k3) {
   constexpr int RF = R > 0 ? 16 * R : 0;
   const double H = s.H;
-  // F0 = f(y); K1 = G^-1 F0
-  stage_fun<RF>(k, y, h2o, m, k1);
   const double ghinv = 1.0 / (H * ROS_GAMMA1);
-  launder(k);
-  launder(y);
-  launder(k1);
-  // G = -J(y) + ghinv*I formed row by row inside its own LU decomposition
-  if (stage_jac_decomp<R>(k, y, h2o, m, ghinv, g) != 0) {
-    s.status = ST_SINGULAR;
-    return;
-  }
   const double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
-  if constexpr (__is_same(VA, WsView)) {
-    // one instance of the solve for the three stages: the stage vector is
-    // picked at run time (a scalar offset), which keeps the code compact
-    int region = k1.region;
-#pragma nounroll
-    for (int st = 0; st < 3; st++) {
-      if (st == 1) {
-        // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
-        launder(k1);
-        launder(y);
-        vmap<16>(k2, [](int, const double* a) { return a[0] + a[1]; }, y, k1);
-        launder(k2);
-        launder(k);
-        stage_fun<RF>(k, k2, h2o, m, f);
-        launder(f);
-        launder(k1);
-        vmap<16>(k2, [hc21](int, const double* a) { return fma(a[0], hc21, a[1]); }, k1, f);
-        region = k2.region;
-      } else if (st == 2) {
-        // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
-        launder(k2);
-        launder(f);
-        launder(k1);
-        vmap<16>(k3, [hc31, hc32](int, const double* a) {
-          return fma(a[0], hc32, fma(a[1], hc31, a[2]));
-        }, k2, k1, f);
-        region = k3.region;
-      }
-      WsView x = k1;
-      x.region = region;
-      launder(x);
-      launder(g);
-      stage_solve(g, x);
+  const double rtol = o.rtol, atol = o.atol;
+  double err = 0.0;
+  if constexpr (is_ws_like<GA>::value) {
+    // Fused stages: each right-hand side is formed in registers and solved
+    // there; K1 and K2 are stored once, K3 never (the combination uses it
+    // in place), and F2 once for the third stage.  The decomposition does
+    // not depend on F0, so it comes first.
+    // G = -J(y) + ghinv*I formed row by row inside its own LU decomposition
+    if (gen::jac_decomp_pf16(k, y, h2o, m, ghinv, g) != 0) {
+      s.status = ST_SINGULAR;
+      return;
     }
+    double x[NVAR];
+    // K1 = G^-1 f(y)
+    gen::fun_pf16_reg(k, y, h2o, m, x);
+    gen::solve_pf16_reg(g, x);
+#pragma unroll
+    for (int i = 0; i < NVAR; i++) WR(k1, i, x[i]);
+    // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
+    vmap<16>(k2, [](int, const double* a) { return a[0] + a[1]; }, y, k1);
+    gen::fun_pf16_reg(k, k2, h2o, m, x);
+#pragma unroll
+    for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
+    xpass<16>([&x, hc21](int i, const double* a) { x[i] = fma(a[0], hc21, x[i]); }, k1);
+    gen::solve_pf16_reg(g, x);
+#pragma unroll
+    for (int i = 0; i < NVAR; i++) WR(k2, i, x[i]);
+    // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
+    xpass<16>([&x, hc31, hc32](int i, const double* a) {
+      x[i] = fma(a[0], hc32, fma(a[1], hc31, a[2]));
+    }, k2, k1, f);
+    gen::solve_pf16_reg(g, x);
+    // combination and WRMS error norm with K3 in registers; Ynew kept in f
+    xpass<8>([&x, &err, f, rtol, atol](int i, const double* a) {
+      const double a1 = a[0], a2 = a[1], a3 = x[i], yi = a[2];
+      const double sm = fma(ROS_M3, a3, fma(ROS_M2, a2, ROS_M1 * a1));
+      const double yn = yi + sm;
+      const double e = fma(ROS_E3, a3, fma(ROS_E2, a2, ROS_E1 * a1));
+      const double ymax = dmax_ref(fabs(yi), fabs(yn));
+      const double q = e / fma(rtol, ymax, atol);
+      err = fma(q, q, err);
+      WR(f, i, yn);
+    }, k1, k2, y);
   } else {
+    // F0 = f(y); K1 = G^-1 F0
+    stage_fun<RF>(k, y, h2o, m, k1);
+    // G = -J(y) + ghinv*I formed row by row inside its own LU decomposition
+    if (stage_jac_decomp<R>(k, y, h2o, m, ghinv, g) != 0) {
+      s.status = ST_SINGULAR;
+      return;
+    }
     stage_solve(g, k1);
     // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
     vmap<16>(k2, [](int, const double* a) { return a[0] + a[1]; }, y, k1);
@@ -252,15 +293,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
       return fma(a[0], hc32, fma(a[1], hc31, a[2]));
     }, k2, k1, f);
     stage_solve(g, k3);
-  }
-  launder(k1);
-  launder(k2);
-  launder(k3);
-  launder(y);
-  // combination and WRMS error norm; Ynew kept in f until the decision
-  double err = 0.0;
-  {
-    const double rtol = o.rtol, atol = o.atol;
+    // combination and WRMS error norm; Ynew kept in f until the decision
     vmap<8>(f, [&err, rtol, atol](int, const double* a) {
       const double a1 = a[0], a2 = a[1], a3 = a[2], yi = a[3];
       const double sm = fma(ROS_M3, a3, fma(ROS_M2, a2, ROS_M1 * a1));
