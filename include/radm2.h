@@ -184,6 +184,33 @@ int radm2_rosenbrock(radm2_ctx *ctx, const radm2_domain *state, double Tstart, d
                      const double *AbsTol, const double *RelTol, double RPAR[20],
                      int32_t IPAR[20]);
 
+/*
+ * WRF-Chem fields -> the solver's inputs over a box of grid cells: the
+ * reference's dev_convert (mgpu.cu:513-646, unit constants mgpu.cu:30-44).
+ * Input arrays are WRF-ordered [field][ny][nz][nx] device arrays:
+ *   phy   [3]:    pressure (Pa), temperature (K), air density (kg/m3)
+ *   moist [>= 2]: field 1 = water vapour mixing ratio qv (kg/kg)
+ *   ph    [>= 21]: photolysis rates (1/min), the reference's jv order
+ *   chem  [>= 44], rad [>= 28]: mixing ratios (ppmv) in WRF's field order
+ * Box cell (j, k, i), 0 <= j < by, 0 <= k < bz, 0 <= i < bx, reads input
+ * (j + oy, k + oz, i + ox) (the reference: 39 x 19 x 39 cells at halo
+ * offsets oy = ox = 4 of a 48 x 20 x 48 grid) and writes output cell
+ * p = (j*bz + k)*bx + i of conc [59][ld], temp, cair, h2o, rc_n2o5 [ld]
+ * and jrate [21][ld].  rc_fixed = 0 reproduces the reference's N2O5
+ * hydrolysis rate as written (mgpu.cu:552, a misplaced parenthesis makes it
+ * ~1e-135 s^-1); 1 uses 1/(3.6e4*exp(-(rh/0.28)^2.8) + 300).  Asynchronous
+ * on `stream` (a hipStream_t; NULL = the default stream).
+ */
+typedef struct radm2_wrf_grid {
+  int nx, ny, nz; /* input grid */
+  int ox, oy, oz; /* box offset in the input grid */
+  int bx, by, bz; /* box size */
+} radm2_wrf_grid;
+int radm2_wrf_convert(void *stream, const radm2_wrf_grid *grid, const double *phy,
+                      const double *moist, const double *ph, const double *chem,
+                      const double *rad, int rc_fixed, int64_t ld, double *conc, double *temp,
+                      double *cair, double *h2o, double *jrate, double *rc_n2o5);
+
 /* Text for a call-level or per-cell code (ros_ErrorMsg, mgpu.cu:976-1005). */
 const char *radm2_strerror(int code);
 

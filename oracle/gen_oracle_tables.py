@@ -59,6 +59,11 @@ def main(dst=None):
     L.append("static const int O_LU_CROW[60] = {%s};" % ",".join(map(str, lu["crow"])))
     L.append("static const int O_LU_DIAG[59] = {%s};" % ",".join(map(str, lu["diag"])))
     L.append("static const int O_LU_ICOL[659] = {%s};" % ",".join(map(str, lu["icol"])))
+    w = mech["wrf"]
+    L.append("static const int O_WRF_SRC[59] = {%s};" % ",".join(str(a) for a, _ in w["var_source"]))
+    L.append("static const int O_WRF_FIELD[59] = {%s};" % ",".join(str(f) for _, f in w["var_source"]))
+    for name, v in sorted(w["constants"].items()):
+        L.append("#define O_%s (%r)" % (name.upper(), v))
     L.append("static void o_rconst_literal(double T, double M, const double *J, double RC_N2O5,"
              " double *k) {")
     for r, e in enumerate(mech["rates"]):

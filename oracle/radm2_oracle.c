@@ -346,3 +346,43 @@ int oracle_integrate(int64_t n, int64_t ld, double *conc, const double *temp,
   }
   return 0;
 }
+
+/* ---- WRF-Chem input adapter: dev_convert (mgpu.cu:513-646) -------------- */
+/* unit constants as mgpu.cu:30-44 defines them */
+#define O_MWH2O (2 * O_MWH + O_MWO)
+#define O_DENS2CON_A (1.0e-3 * (1.0 / O_MWAIR) * O_NAVGDRO)
+#define O_DENS2CON_W (1.0e-3 * (1.0 / O_MWH2O) * O_NAVGDRO)
+
+void oracle_wrf_convert(const int *g, const double *phy, const double *moist, const double *ph,
+                        const double *chem, const double *rad, int rc_fixed, int64_t ld,
+                        double *conc, double *temp, double *cair, double *h2o, double *jrate,
+                        double *rc_n2o5) {
+  const int nx = g[0], ny = g[1], nz = g[2], ox = g[3], oy = g[4], oz = g[5];
+  const int bx = g[6], by = g[7], bz = g[8];
+  const int64_t fs = (int64_t)nx * ny * nz;
+  for (int j = 0; j < by; j++)
+    for (int k = 0; k < bz; k++)
+      for (int i = 0; i < bx; i++) {
+        /* output p = j*gridDim.x*blockDim.x + k*blockDim.x + i (mgpu.cu:522),
+         * input at the halo-shifted (j+oy, k+oz, i+ox) (mgpu.cu:532-535) */
+        const int64_t p = ((int64_t)j * bz + k) * bx + i;
+        const int64_t q = ((int64_t)(j + oy) * nz + (k + oz)) * nx + (i + ox);
+        const double P = phy[q], T = phy[fs + q], rho = phy[2 * fs + q], qv = moist[fs + q];
+        cair[p] = O_DENS2CON_A * rho;                        /* mgpu.cu:539 */
+        h2o[p] = O_DENS2CON_W * qv * rho;                    /* mgpu.cu:542 */
+        temp[p] = T;                                         /* mgpu.cu:545 */
+        /* mgpu.cu:548-552 */
+        const double es = 1000. * 0.6112 * exp(17.67 * (T - 273.15) / (T - 29.65));
+        const double qvs = es / (P - es);
+        double rh = qv / qvs;
+        rh = O_MIN(O_MAX(rh, 0.0), 1.0);
+        rc_n2o5[p] = rc_fixed ? 1.0 / (3.6e4 * exp(-pow(rh / 0.28, 2.8)) + 300.0)
+                              : 1.0 / (3.6e4 * exp(-pow((rh / 0.28), 2.8) + 300.0));
+        const double conv = 1.0E-6 * O_DENS2CON_A * rho;    /* mgpu.cu:555 */
+        for (int s = 0; s < O_NJ; s++) jrate[s * ld + p] = ph[s * fs + q] / 60.0;  /* 557-584 */
+        for (int s = 0; s < O_NVAR; s++) {                    /* mgpu.cu:587-645 */
+          const double *a = O_WRF_SRC[s] ? rad : chem;
+          conc[s * ld + p] = conv * O_MAX(a[O_WRF_FIELD[s] * fs + q], 0.0);
+        }
+      }
+}

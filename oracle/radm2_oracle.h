@@ -58,6 +58,14 @@ int oracle_integrate(int64_t n, int64_t ld, double *conc, const double *temp,
                      int32_t *nacc, int32_t *nrej, int32_t *status, double *hexit,
                      int nthreads);
 
+/* WRF-Chem fields -> solver inputs over a box (dev_convert, mgpu.cu:513-646);
+ * grid = {nx, ny, nz, ox, oy, oz, bx, by, bz}, arrays [field][ny][nz][nx],
+ * outputs as the product's radm2_wrf_convert. */
+void oracle_wrf_convert(const int *grid, const double *phy, const double *moist, const double *ph,
+                        const double *chem, const double *rad, int rc_fixed, int64_t ld,
+                        double *conc, double *temp, double *cair, double *h2o, double *jrate,
+                        double *rc_n2o5);
+
 #ifdef __cplusplus
 }
 #endif

@@ -692,6 +692,24 @@ def gen_hot_tables(mech):
     return "\n".join(out)
 
 
+def gen_wrf_tables(mech):
+    """dev_convert's species shuffle (mgpu.cu:585-645) and unit constants
+    (mgpu.cu:30-44): WRF source array (0 chemistry, 1 radicals) and field
+    per variable species."""
+    w = mech["wrf"]
+    c = w["constants"]
+    src = [a for a, _ in w["var_source"]]
+    fld = [f for _, f in w["var_source"]]
+    return "\n".join([
+        "// WRF-Chem input shuffle: source array (0 chemistry, 1 radicals) and field per species",
+        "constexpr int WRF_SRC[59] = {%s};" % ",".join(map(str, src)),
+        "constexpr int WRF_FIELD[59] = {%s};" % ",".join(map(str, fld)),
+        "constexpr double WRF_NAVGDRO = %r;" % c["navgdro"],
+        "constexpr double WRF_MWAIR = %r;" % c["mwair"],
+        "constexpr double WRF_MWH2O = 2 * %r + %r;" % (c["mwh"], c["mwo"]),
+    ])
+
+
 def generate(dst=None, mech_path=None):
     mech = load_mech(mech_path)
     dst = dst or os.path.join(PKG, "csrc", "generated", "radm2_gen.h")
@@ -757,6 +775,8 @@ def generate(dst=None, mech_path=None):
         gen_fun_pf_reg(mech, 16),
         "",
         gen_hot_tables(mech),
+        "",
+        gen_wrf_tables(mech),
         "",
         "}  // namespace gen",
         "}  // namespace radm2",

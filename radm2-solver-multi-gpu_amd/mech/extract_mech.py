@@ -192,6 +192,35 @@ def symbolic_lu(pattern, n=59):
     return [(i, j) for i in range(n) for j in sorted(rows[i])]
 
 
+def parse_wrf_map(src):
+    """dev_convert's species shuffle (mgpu.cu:585-645): for each variable
+    species, the WRF source array (0 = the chemistry array, 1 = the
+    radical array) and its field; the chemistry fields' P_* names
+    (mgpu.cu:81-122); and the molecular-weight constants (mgpu.cu:30-44)."""
+    text = "\n".join(src[512:646])
+    pairs = re.findall(r"VAR\[MY_I\((\d+)\)\]\s*=\s*conv \* MAX\(\((wrfchemdata|wrfraddata)\[\(\((\d+)\)",
+                       text)
+    vmap = {}
+    for v, arr, fld in pairs:
+        assert int(v) not in vmap
+        vmap[int(v)] = [0 if arr == "wrfchemdata" else 1, int(fld)]
+    assert sorted(vmap) == list(range(59)), sorted(set(range(59)) - set(vmap))
+    pnames = {}
+    for ln in src[80:123]:
+        mo = re.match(r"#define P_(\w+)\s+\((\d+)\)", ln.strip())
+        if mo:
+            pnames[mo.group(1)] = int(mo.group(2))
+    consts = {}
+    for name in ("navgdro", "mwh", "mwo", "mwair"):
+        for ln in src[25:50]:
+            mo = re.match(r"#define %s\s+\(([-0-9.eE+]+)\)" % name, ln.strip())
+            if mo:
+                consts[name] = float(mo.group(1))
+    assert set(consts) == {"navgdro", "mwh", "mwo", "mwair"}, consts
+    return {"var_source": [vmap[v] for v in range(59)], "chem_names": pnames,
+            "constants": consts, "nph": 28, "halo": [4, 0, 4]}
+
+
 def main(path):
     src = open(path).read().split("\n")
     species, fixed = parse_species(src)
@@ -274,10 +303,12 @@ def main(path):
         "jac_nonzero": len(jpat),
         "lu": {"crow": crow, "diag": diag, "icol": icol, "nnz": 659},
         "jac_written_zero": zero_written,
+        "wrf": parse_wrf_map(src),
         "provenance": {
             "species": "mgpu.cu:125-189", "rates": "mgpu.cu:763-918",
             "monomials": "mgpu.cu:1103-1258", "tendency": "mgpu.cu:1260-1385",
             "jacobian": "mgpu.cu:1685-2651", "lu_pattern": "mgpu.cu:2873-3090",
+            "wrf": "mgpu.cu:30-44, 81-122, 513-646",
         },
     }
     dst = os.path.join(HERE, "radm2.json")

@@ -75,6 +75,11 @@ class Domain(ctypes.Structure):
                 ("opts", ctypes.POINTER(Opts))]
 
 
+class WrfGrid(ctypes.Structure):
+    """radm2_wrf_grid: input grid (nx, ny, nz), box offset and box size."""
+    _fields_ = [(n, ctypes.c_int) for n in ("nx", "ny", "nz", "ox", "oy", "oz", "bx", "by", "bz")]
+
+
 _EXPORTS = [
     ("radm2_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_void_p]),
     ("radm2_destroy", None, [ctypes.c_void_p]),
@@ -99,6 +104,9 @@ _EXPORTS = [
       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)]),
     ("radm2_strerror", ctypes.c_char_p, [ctypes.c_int]),
+    ("radm2_wrf_convert", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(WrfGrid)] + [ctypes.c_void_p] * 5
+     + [ctypes.c_int, ctypes.c_int64] + [ctypes.c_void_p] * 6),
 ]
 EXPORTED_SYMBOLS = tuple(n for n, _, _ in _EXPORTS)
 

@@ -142,3 +142,20 @@ def integrate(cells, dt, rtol, atol, opts=None, h0=None, nthreads=0, n=None, use
         ctypes.byref(opts) if opts else None, _p(h0a), _p(ns), _p(na), _p(nr), _p(st), _p(hx),
         nthreads)
     return conc, dict(nstep=ns, nacc=na, nrej=nr, status=st, hexit=hx), code
+
+
+def wrf_convert(grid, phy, moist, ph, chem, rad, rc_fixed=False, ld=None):
+    """oracle_wrf_convert: grid = (nx, ny, nz, ox, oy, oz, bx, by, bz), WRF
+    arrays [field][ny][nz][nx] (float64); returns the solver inputs."""
+    L = lib()
+    L.oracle_wrf_convert.restype = None
+    g = np.ascontiguousarray(grid, dtype=np.int32)
+    n = int(g[6]) * int(g[7]) * int(g[8])
+    ld = n if ld is None else ld
+    out = dict(conc=np.zeros((59, ld)), temp=np.zeros(ld), cair=np.zeros(ld), h2o=np.zeros(ld),
+               jrate=np.zeros((21, ld)), rc_n2o5=np.zeros(ld))
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (phy, moist, ph, chem, rad)]
+    L.oracle_wrf_convert(_p(g), *[_p(a) for a in arrs], ctypes.c_int(1 if rc_fixed else 0),
+                         ctypes.c_int64(ld), _p(out["conc"]), _p(out["temp"]), _p(out["cair"]),
+                         _p(out["h2o"]), _p(out["jrate"]), _p(out["rc_n2o5"]))
+    return out
