@@ -235,7 +235,9 @@ int oracle_resolve(double dt, double rtol, double atol, const oracle_opts_in *in
   o->facsafe = in->facsafe == 0.0 ? 0.9 : in->facsafe;
   if (in->facsafe < 0.0) return -4;
   if (atol <= 0.0 || rtol <= 10.0 * o->roundoff || rtol >= 1.0) return -5;
-  if (method != 2) return -2; /* only Ros3 is implemented (mgpu.cu:3802-3811) */
+  /* the reference initialises only Ros3 (mgpu.cu:3802-3811); KPP's other
+   * four methods are implemented here */
+  o->method = method;
   return 0;
 }
 
@@ -314,6 +316,181 @@ done:
   return status;
 }
 
+/* ---- KPP's Rosenbrock family, generic stage loop ----------------------------
+ * Coefficients of KPP's methods by IPAR[3] number (1 Ros2, 2 Ros3, 3 Ros4,
+ * 4 Rodas3, 5 Rodas4) as published: Sandu, Verwer, Blom, Spee, Carmichael
+ * and Potra (1997), "Benchmarking stiff ODE solvers for atmospheric
+ * chemistry problems II: Rosenbrock solvers", Atmos. Environ. 31, and
+ * Hairer and Wanner (1996), Solving ODEs II (RODAS).  A and C are the
+ * strictly lower triangles packed row by row: entry (i, j), j < i, at
+ * i*(i-1)/2 + j.  The reference implements only Ros3 (mgpu.cu:1008-1062,
+ * 3802-3811).  Stage scheme (the reference's commented stage loop,
+ * mgpu.cu:3595-3625, in KPP's general form):
+ *   F = f(y) for stage 0; F = f(y + sum_j A(s,j) K_j) when NewF(s), else
+ *   the previous stage's F; K_s = G^-1 (F + sum_j C(s,j)/H K_j);
+ *   Ynew = y + sum M_j K_j, error sum E_j K_j.  Autonomous only. */
+typedef struct {
+  int S;
+  double A[15], C[15], M[6], E[6], gamma1, elo;
+  int newf[6];
+} o_method;
+
+static o_method o_method_table(int method) {
+  o_method r;
+  memset(&r, 0, sizeof r);
+  if (method == 1) { /* Ros2 */
+    const double g = 1.0 + 1.0 / sqrt(2.0);
+    r.S = 2;
+    r.A[0] = 1.0 / g;
+    r.C[0] = -2.0 / g;
+    r.newf[0] = r.newf[1] = 1;
+    r.M[0] = 3.0 / (2.0 * g);
+    r.M[1] = 1.0 / (2.0 * g);
+    r.E[0] = 1.0 / (2.0 * g);
+    r.E[1] = 1.0 / (2.0 * g);
+    r.gamma1 = g;
+    r.elo = 2.0;
+  } else if (method == 2) { /* Ros3 */
+    r.S = 3;
+    r.A[0] = 1.0; r.A[1] = 1.0; r.A[2] = 0.0;
+    r.C[0] = ROS_C21; r.C[1] = ROS_C31; r.C[2] = ROS_C32;
+    r.newf[0] = 1; r.newf[1] = 1; r.newf[2] = 0;
+    r.M[0] = ROS_M1; r.M[1] = ROS_M2; r.M[2] = ROS_M3;
+    r.E[0] = ROS_E1; r.E[1] = ROS_E2; r.E[2] = ROS_E3;
+    r.gamma1 = ROS_GAMMA1;
+    r.elo = ROS_ELO;
+  } else if (method == 3) { /* Ros4 */
+    r.S = 4;
+    r.A[0] = 2.0; r.A[1] = 1.867943637803922; r.A[2] = 0.2344449711399156;
+    r.A[3] = r.A[1]; r.A[4] = r.A[2]; r.A[5] = 0.0;
+    r.C[0] = -7.137615036412310; r.C[1] = 2.580708087951457; r.C[2] = 0.6515950076447975;
+    r.C[3] = -2.137148994382534; r.C[4] = -0.3214669691237626; r.C[5] = -0.6949742501781779;
+    r.newf[0] = r.newf[1] = r.newf[2] = 1; r.newf[3] = 0;
+    r.M[0] = 2.255570073418735; r.M[1] = 0.2870493262186792;
+    r.M[2] = 0.4353179431840180; r.M[3] = 1.093502252409163;
+    r.E[0] = -0.2815431932141155; r.E[1] = -0.07276199124938920;
+    r.E[2] = -0.1082196201495311; r.E[3] = -1.093502252409163;
+    r.gamma1 = 0.5728200000000000;
+    r.elo = 4.0;
+  } else if (method == 4) { /* Rodas3 */
+    r.S = 4;
+    r.A[0] = 0.0; r.A[1] = 2.0; r.A[2] = 0.0; r.A[3] = 2.0; r.A[4] = 0.0; r.A[5] = 1.0;
+    r.C[0] = 4.0; r.C[1] = 1.0; r.C[2] = -1.0; r.C[3] = 1.0; r.C[4] = -1.0; r.C[5] = -(8.0 / 3.0);
+    r.newf[0] = 1; r.newf[1] = 0; r.newf[2] = 1; r.newf[3] = 1;
+    r.M[0] = 2.0; r.M[1] = 0.0; r.M[2] = 1.0; r.M[3] = 1.0;
+    r.E[3] = 1.0;
+    r.gamma1 = 0.5;
+    r.elo = 3.0;
+  } else { /* Rodas4 */
+    r.S = 6;
+    const double a[15] = {1.544000000000000, 0.9466785280815826, 0.2557011698983284,
+                          3.314825187068521, 2.896124015972201, 0.9986419139977817,
+                          1.221224509226641, 6.019134481288629, 12.53708332932087,
+                          -0.6878860361058950, 1.221224509226641, 6.019134481288629,
+                          12.53708332932087, -0.6878860361058950, 1.0};
+    const double c[15] = {-5.668800000000000, -2.430093356833875, -0.2063599157091915,
+                          -0.1073529058151375, -9.594562251023355, -20.47028614809616,
+                          7.496443313967647, -10.24680431464352, -33.99990352819905,
+                          11.70890893206160, 8.083246795921522, -7.981132988064893,
+                          -31.52159432874371, 16.31930543123136, -6.058818238834054};
+    memcpy(r.A, a, sizeof a);
+    memcpy(r.C, c, sizeof c);
+    r.M[0] = a[6]; r.M[1] = a[7]; r.M[2] = a[8]; r.M[3] = a[9]; r.M[4] = 1.0; r.M[5] = 1.0;
+    r.E[5] = 1.0;
+    for (int i = 0; i < 6; i++) r.newf[i] = 1;
+    r.gamma1 = 0.25;
+    r.elo = 4.0;
+  }
+  return r;
+}
+
+int oracle_method_stages(int method) { return o_method_table(method).S; }
+
+int oracle_ros_cell(int method, double *y, const double *k, double h2o, double m, double dt,
+                    double rtol, double atol, const oracle_opts *o, double h0, int32_t *nstep,
+                    int32_t *nacc, int32_t *nrej, double *hexit) {
+  const o_method r = o_method_table(method);
+  double G[O_LU_NNZ], F0[O_NVAR], F[O_NVAR], K[6][O_NVAR], yn[O_NVAR];
+  const double Tend = dt, rnd = o->roundoff;
+  double T = 0.0;
+  double H = O_MIN(o->hstart, o->hmax);
+  if (fabs(H) <= 10.0 * rnd) H = DELTA_MIN;
+  if (h0 > 0.0) {
+    H = O_MIN(O_MIN(h0, dt), o->hmax);
+    if (fabs(H) <= 10.0 * rnd) H = DELTA_MIN;
+  }
+  int rejLast = 0, rejMore = 0, status = 1;
+  int32_t ns = 0, na = 0, nr = 0;
+  while ((T - Tend) + rnd <= 0.0) {
+    int first = 1;
+    oracle_fun(y, h2o, m, k, F0);
+    for (;;) {
+      if (ns > o->max_steps) { status = -6; goto done; }
+      if (T + 0.1 * H == T || !(H > rnd)) { status = -7; goto done; }
+      if (first) { H = O_MIN(H, fabs(Tend - T)); first = 0; }
+      const double ghinv = 1.0 / (H * r.gamma1);
+      oracle_jac_prep(y, h2o, m, k, ghinv, G);
+      if (oracle_decomp(G, 0) != 0) { status = -8; goto done; }
+      for (int st = 0; st < r.S; st++) {
+        const int base = st * (st - 1) / 2;
+        if (st == 0) {
+          memcpy(F, F0, sizeof F);
+        } else if (r.newf[st]) {
+          for (int i = 0; i < O_NVAR; i++) {
+            double v = y[i];
+            for (int j = 0; j < st; j++) v = fma(r.A[base + j], K[j][i], v);
+            yn[i] = v;
+          }
+          oracle_fun(yn, h2o, m, k, F);
+        }
+        for (int i = 0; i < O_NVAR; i++) {
+          double v = F[i];
+          for (int j = 0; j < st; j++) v = fma(K[j][i], r.C[base + j] / H, v);
+          K[st][i] = v;
+        }
+        oracle_solve(G, K[st], 0);
+      }
+      double err = 0.0;
+      for (int i = 0; i < O_NVAR; i++) {
+        double sm = r.M[0] * K[0][i], e = r.E[0] * K[0][i];
+        for (int j = 1; j < r.S; j++) {
+          sm = fma(r.M[j], K[j][i], sm);
+          e = fma(r.E[j], K[j][i], e);
+        }
+        yn[i] = y[i] + sm;
+        double ymax = O_MAX(fabs(y[i]), fabs(yn[i]));
+        double q = e / fma(rtol, ymax, atol);
+        err = fma(q, q, err);
+      }
+      err = sqrt(err / (double)O_NVAR);
+      double fac = O_MIN(o->facmax, O_MAX(o->facmin, o->facsafe / pow(err, 1.0 / r.elo)));
+      double Hnew = H * fac;
+      ns++;
+      if (err <= 1.0 || H <= o->hmin) {
+        for (int i = 0; i < O_NVAR; i++) y[i] = yn[i];
+        T = T + H;
+        if (rejLast) Hnew = O_MIN(Hnew, H);
+        rejLast = 0;
+        rejMore = 0;
+        H = Hnew;
+        na++;
+        break;
+      }
+      if (rejMore) Hnew = H * o->facrej;
+      rejMore = rejLast;
+      rejLast = 1;
+      H = Hnew;
+      nr++;
+    }
+  }
+done:
+  if (nstep) *nstep = ns;
+  if (nacc) *nacc = na;
+  if (nrej) *nrej = nr;
+  if (hexit) *hexit = H;
+  return status;
+}
+
 int oracle_integrate(int64_t n, int64_t ld, double *conc, const double *temp,
                      const double *cair, const double *h2o, const double *jrate,
                      const double *rc_n2o5, double dt, double rtol, double atol,
@@ -335,7 +512,9 @@ int oracle_integrate(int64_t n, int64_t ld, double *conc, const double *temp,
     for (int i = 0; i < O_NVAR; i++) y[i] = conc[i * ld + p];
     int32_t a = 0, b = 0, c = 0;
     double hx = 0.0;
-    int st = oracle_ros3_cell(y, k, h2o[p], cair[p], dt, rtol, atol, &o, h0 ? h0[p] : 0.0, &a,
+    int st = o.method != 2 ? oracle_ros_cell(o.method, y, k, h2o[p], cair[p], dt, rtol, atol, &o,
+                                             h0 ? h0[p] : 0.0, &a, &b, &c, &hx)
+                           : oracle_ros3_cell(y, k, h2o[p], cair[p], dt, rtol, atol, &o, h0 ? h0[p] : 0.0, &a,
                               &b, &c, &hx);
     for (int i = 0; i < O_NVAR; i++) conc[i * ld + p] = y[i];
     if (nstep) nstep[p] = a;

@@ -28,7 +28,7 @@ typedef struct {
 
 typedef struct {
   double hmin, hmax, hstart, facmin, facmax, facrej, facsafe, roundoff;
-  int32_t max_steps;
+  int32_t max_steps, method; /* method: KPP number 1..5 */
 } oracle_opts;
 
 /* building blocks, one cell */
@@ -65,6 +65,13 @@ void oracle_wrf_convert(const int *grid, const double *phy, const double *moist,
                         const double *chem, const double *rad, int rc_fixed, int64_t ld,
                         double *conc, double *temp, double *cair, double *h2o, double *jrate,
                         double *rc_n2o5);
+
+/* KPP's Rosenbrock family by IPAR[3] number (1 Ros2, 2 Ros3, 3 Ros4,
+ * 4 Rodas3, 5 Rodas4): one cell with the generic stage loop. */
+int oracle_ros_cell(int method, double *y, const double *k, double h2o, double m, double dt,
+                    double rtol, double atol, const oracle_opts *o, double h0, int32_t *nstep,
+                    int32_t *nacc, int32_t *nrej, double *hexit);
+int oracle_method_stages(int method);
 
 #ifdef __cplusplus
 }

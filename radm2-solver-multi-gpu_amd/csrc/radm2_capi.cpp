@@ -80,19 +80,21 @@ constexpr int kDefaultRefillMin = 1;
 constexpr int kDefaultWaves = 1;
 constexpr int kDefaultPlacement = radm2::PLACE_HOT160R128;
 
-const radm2::KernelVariant* pick_variant(const radm2_ctx* c) {
+// The configured kernel variant; family: one that runs all of KPP's
+// Rosenbrock methods (built for the default placement only).
+const radm2::KernelVariant* pick_variant(const radm2_ctx* c, bool family) {
   const int w = c->waves ? c->waves : kDefaultWaves;
   const int pl = c->placement >= 0 ? c->placement : kDefaultPlacement;
   const radm2::KernelVariant* v = radm2::kernel_variants();
   for (int i = 0; i < radm2::kNumVariants; i++)
-    if (v[i].waves == w && v[i].placement == pl) return &v[i];
+    if (v[i].waves == w && v[i].placement == pl && v[i].family == (int)family) return &v[i];
   return nullptr;
 }
 
 // Workspace for `waves` wave slots, grown (never shrunk) on demand.  The
 // stream is drained first: a running kernel may still use the old buffer.
-int ensure_ws(hipStream_t st, double** ws, size_t* have, int64_t waves) {
-  const size_t need = (size_t)waves * radm2::WS_DOUBLES_PER_LANE * radm2::WAVE * sizeof(double);
+int ensure_ws(hipStream_t st, double** ws, size_t* have, int64_t waves, int ws_doubles) {
+  const size_t need = (size_t)waves * ws_doubles * radm2::WAVE * sizeof(double);
   if (need <= *have) return 0;
   HIP_TRY(hipStreamSynchronize(st));
   if (*ws) HIP_TRY(hipFree(*ws));
@@ -137,8 +139,9 @@ int resolve_opts(double dt, double rtol, double atol, const radm2_opts* in, Ros3
   o->facsafe = in->facsafe == 0.0 ? 0.9 : in->facsafe;
   if (in->facsafe < 0.0) return -4;
   if (atol <= 0.0 || rtol <= 10.0 * o->roundoff || rtol >= 1.0) return -5;
-  if (method != 2) return -2;  // only Ros3 (mgpu.cu:3802-3811)
-  o->pad = 0;
+  // the reference initialises only Ros3 (mgpu.cu:3802-3811); KPP's other
+  // four methods run through the generic stage loop
+  o->method = method;
   return 0;
 }
 
@@ -198,7 +201,7 @@ int pick_block(const radm2_ctx* c) {
 // grow = false for graph capture, where the plan sized it beforehand.
 int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
             unsigned long long* counter, hipStream_t st, double** ws, size_t* ws_bytes,
-            bool grow = true) {
+            bool family, bool grow = true) {
   LaunchArgs a;
   a.doms = d_doms;
   a.ndom = ndom;
@@ -207,8 +210,8 @@ int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
   a.stats = c->stats_on ? c->d_stats : nullptr;
   a.refill_min = c->refill_min ? c->refill_min : kDefaultRefillMin;
   const int block = pick_block(c);
-  const radm2::KernelVariant* v = pick_variant(c);
-  if (!v) return RADM2_E_ARG;
+  const radm2::KernelVariant* v = pick_variant(c, family);
+  if (!v) return family ? RADM2_E_UNSUPPORTED : RADM2_E_ARG;
   if (v->placement >= radm2::PLACE_HOT160 && block != 64) return RADM2_E_ARG;
   int64_t blocks = 0, waves = 0;
   if (c->schedule == RADM2_SCHED_GRID) {
@@ -224,8 +227,8 @@ int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
   }
   if (v->placement >= radm2::PLACE_WS) {
     if (grow) {
-      if (int rc = ensure_ws(st, ws, ws_bytes, waves)) return rc;
-    } else if (*ws_bytes < (size_t)waves * radm2::WS_DOUBLES_PER_LANE * radm2::WAVE * sizeof(double)) {
+      if (int rc = ensure_ws(st, ws, ws_bytes, waves, v->ws_doubles)) return rc;
+    } else if (*ws_bytes < (size_t)waves * v->ws_doubles * radm2::WAVE * sizeof(double)) {
       return RADM2_E_ARG;
     }
     a.ws = *ws;
@@ -243,15 +246,22 @@ int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
   return 0;
 }
 
-int build_table(const radm2_domain* doms, int ndom, std::vector<DomainDesc>& out,
-                int64_t* total) {
+// Device table of the domains.  *family: some domain uses a method other
+// than Ros3, which only the family variant runs (RADM2_E_UNSUPPORTED when the
+// configured placement has none).
+int build_table(const radm2_ctx* c, const radm2_domain* doms, int ndom,
+                std::vector<DomainDesc>& out, int64_t* total, bool* family) {
   out.resize(ndom);
   int64_t first = 0;
+  bool other_methods = false;
   for (int i = 0; i < ndom; i++) {
     if (int rc = fill_desc(doms[i], first, &out[i])) return rc;
+    other_methods |= out[i].opts.method != 2 && doms[i].ncells > 0;
     first += doms[i].ncells;
   }
   *total = first;
+  *family = other_methods;
+  if (other_methods && !pick_variant(c, true)) return RADM2_E_UNSUPPORTED;
   return 0;
 }
 
@@ -333,7 +343,8 @@ int radm2_set_kernel(radm2_ctx* c, int placement, int waves_per_simd) {
   const int w = waves_per_simd ? waves_per_simd : kDefaultWaves;
   const radm2::KernelVariant* v = radm2::kernel_variants();
   bool found = false;
-  for (int i = 0; i < radm2::kNumVariants; i++) found |= (v[i].placement == pl && v[i].waves == w);
+  for (int i = 0; i < radm2::kNumVariants; i++)
+    found |= (v[i].placement == pl && v[i].waves == w && v[i].family == 0);
   if (!found) return RADM2_E_UNSUPPORTED;
   c->placement = placement;
   c->waves = waves_per_simd;
@@ -344,7 +355,8 @@ int radm2_integrate_batched(radm2_ctx* c, int ndom, const radm2_domain* doms) {
   if (!c || ndom < 0 || (ndom > 0 && !doms)) return RADM2_E_ARG;
   std::vector<DomainDesc> tab;
   int64_t total = 0;
-  if (int rc = build_table(doms, ndom, tab, &total)) return rc;
+  bool family = false;
+  if (int rc = build_table(c, doms, ndom, tab, &total, &family)) return rc;
   c->timed = false;
   if (total == 0) return RADM2_OK;
   HIP_TRY(hipSetDevice(c->device));
@@ -356,7 +368,8 @@ int radm2_integrate_batched(radm2_ctx* c, int ndom, const radm2_domain* doms) {
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipEventRecord(c->ev_staged, c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  if (int rc = enqueue(c, c->d_doms, ndom, total, c->d_counter, c->stream, &c->d_ws, &c->ws_bytes))
+  if (int rc = enqueue(c, c->d_doms, ndom, total, c->d_counter, c->stream, &c->d_ws, &c->ws_bytes,
+                       family))
     return rc;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   c->timed = true;
@@ -396,7 +409,8 @@ int radm2_plan_create(radm2_ctx* c, int ndom, const radm2_domain* doms, radm2_pl
   *out = nullptr;
   std::vector<DomainDesc> tab;
   int64_t total = 0;
-  if (int rc = build_table(doms, ndom, tab, &total)) return rc;
+  bool family = false;
+  if (int rc = build_table(c, doms, ndom, tab, &total, &family)) return rc;
   HIP_TRY(hipSetDevice(c->device));
   radm2_plan* p = new radm2_plan();
   p->ctx = c;
@@ -418,20 +432,21 @@ int radm2_plan_create(radm2_ctx* c, int ndom, const radm2_domain* doms, radm2_pl
   // size the plan's own workspace before capture (no allocation inside it)
   {
     const int block = pick_block(c);
-    const radm2::KernelVariant* v = pick_variant(c);
+    const radm2::KernelVariant* v = pick_variant(c, family);
     int64_t waves = (total + block - 1) / block * (block / radm2::WAVE);
     if (v && c->schedule == RADM2_SCHED_PERSISTENT) {
       const int per_cu = v->blocks_per_cu(block);
       waves = std::min<int64_t>(waves, (int64_t)std::max(per_cu, 1) * c->cu_count * (block / radm2::WAVE));
     }
-    if (v && v->placement >= radm2::PLACE_WS && ensure_ws(cap, &p->d_ws, &p->ws_bytes, waves) != 0) {
+    if (v && v->placement >= radm2::PLACE_WS && ensure_ws(cap, &p->d_ws, &p->ws_bytes, waves, v->ws_doubles) != 0) {
       hipStreamDestroy(cap);
       return fail(last_hip_error());
     }
   }
   e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    int rc = enqueue(c, p->d_doms, ndom, total, p->d_counter, cap, &p->d_ws, &p->ws_bytes, false);
+    int rc = enqueue(c, p->d_doms, ndom, total, p->d_counter, cap, &p->d_ws, &p->ws_bytes, family,
+                     false);
     hipError_t e2 = hipStreamEndCapture(cap, &p->graph);
     if (rc != 0 && e2 == hipSuccess) e2 = last_hip_error();
     e = e2;
@@ -574,15 +589,18 @@ int radm2_rosenbrock(radm2_ctx* c, const radm2_domain* state, double Tstart, dou
       if (hst[p] != 1 && worst == 1) worst = hst[p];
       hmax_exit = std::max(hmax_exit, hhx[p]);
     }
-    // KPP accounting: one Fun+Jac per time step (accepted attempt), one
-    // more Fun, one Decomp and three Solves per attempt
-    IPAR[10] += (int32_t)(nacc + nstp);
+    // KPP accounting: one Fun+Jac per time step (accepted attempt); per
+    // attempt one Decomp, one Solve per stage and one Fun per stage after
+    // the first that evaluates a new F (Ros2 1, Ros3 1, Ros4 2, Rodas3 2,
+    // Rodas4 5)
+    static const int kNewF[6] = {0, 1, 1, 2, 2, 5}, kStages[6] = {0, 2, 3, 4, 4, 6};
+    IPAR[10] += (int32_t)(nacc + kNewF[ro.method] * nstp);
     IPAR[11] += (int32_t)nacc;
     IPAR[12] += (int32_t)nstp;
     IPAR[13] += (int32_t)nacc;
     IPAR[14] += (int32_t)nrej;
     IPAR[15] += (int32_t)nstp;
-    IPAR[16] += (int32_t)(3 * nstp);
+    IPAR[16] += (int32_t)(kStages[ro.method] * nstp);
     IPAR[17] += (int32_t)nsng;
     RPAR[10] = worst == 1 ? Tend : Tstart;
     RPAR[11] = hmax_exit;

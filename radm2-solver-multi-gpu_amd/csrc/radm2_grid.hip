@@ -20,10 +20,22 @@ namespace radm2 {
 #ifndef RADM2_LROWS
 #define RADM2_LROWS 0
 #endif
+// RADM2_METHODS: 0 Ros3 alone, 1 every method of KPP's Rosenbrock family
+#ifndef RADM2_METHODS
+#define RADM2_METHODS 0
+#endif
 #define RADM2_CAT2(a, b) a##b
 #define RADM2_CAT(a, b) RADM2_CAT2(a, b)
-#define RADM2_SUFFIX \
+#define RADM2_SUFFIX0 \
   RADM2_CAT(RADM2_CAT(RADM2_CAT(_p, RADM2_PLACE), RADM2_CAT(_w, RADM2_WAVES)), RADM2_CAT(_r, RADM2_LROWS))
+#if RADM2_METHODS
+#define RADM2_SUFFIX RADM2_CAT(RADM2_SUFFIX0, _m1)
+#if RADM2_PLACE == RADM2_PLACE_PRIVATE
+#error "the family kernels need the workspace's stage regions"
+#endif
+#else
+#define RADM2_SUFFIX RADM2_SUFFIX0
+#endif
 #define RADM2_NAME(x) RADM2_CAT(x, RADM2_SUFFIX)
 
 // RADM2_WAVES: minimum resident waves per SIMD the register budget must allow
@@ -50,8 +62,9 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
   double kk[NREACT], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
 #else
   const int64_t wslot = __builtin_amdgcn_readfirstlane((int)(g / WAVE));
-  constexpr int kWsBytes = WS_DOUBLES_PER_LANE * WAVE * 8;
-  double* wbase = a.ws + wslot * ((int64_t)WS_DOUBLES_PER_LANE * WAVE);
+  constexpr int kWsDoubles = RADM2_METHODS ? WS_DOUBLES_FAMILY : WS_DOUBLES_ROS3;
+  constexpr int kWsBytes = kWsDoubles * WAVE * 8;
+  double* wbase = a.ws + wslot * ((int64_t)kWsDoubles * WAVE);
   const int wlane = threadIdx.x & (WAVE - 1);
   WsView f = make_view(wbase, WS_F, wlane, kWsBytes);
   WsView k1 = make_view(wbase, WS_K1, wlane, kWsBytes);
@@ -85,7 +98,7 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
     }
     if (run) {
       cell_attempt_top(s, o);
-      if (s.status == ST_RUNNING) ros3_attempt<RADM2_LROWS>(s, o, kk, y, h2o, m, lu, f, k1, k2, k3);
+      if (s.status == ST_RUNNING) ros3_attempt<RADM2_LROWS, RADM2_METHODS != 0>(s, o, kk, y, h2o, m, lu, f, k1, k2, k3);
     }
   }
   if (valid) cell_store(d, p, y, s, s.hfirst, a.stats);

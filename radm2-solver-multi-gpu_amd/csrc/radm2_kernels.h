@@ -39,7 +39,7 @@ struct LaunchArgs {
   int64_t total;           // sum of domain sizes
   unsigned long long* counter;  // persistent: global work counter (zeroed before launch)
   unsigned long long* stats;    // [4] attempts, accepted, rejected, refills (nullable)
-  double* ws;                   // explicit workspace (WS placement), WS_DOUBLES_PER_LANE per lane
+  double* ws;                   // explicit workspace (WS placement), ws_doubles per lane
   int64_t ws_waves;             // wave slots the workspace holds
 };
 
@@ -52,7 +52,11 @@ constexpr int WS_F = WS_LU + LU_NNZ;
 constexpr int WS_K1 = WS_F + NVAR;
 constexpr int WS_K2 = WS_K1 + NVAR;
 constexpr int WS_K3 = WS_K2 + NVAR;
-constexpr int WS_DOUBLES_PER_LANE = WS_K3 + NVAR;
+// doubles per lane: the Ros3 kernels hold K1..K3; the kernels of KPP's
+// whole Rosenbrock family six stage vectors (Rodas4), stage s at
+// WS_K1 + s*NVAR
+constexpr int WS_DOUBLES_ROS3 = WS_K3 + NVAR;
+constexpr int WS_DOUBLES_FAMILY = WS_K1 + 6 * NVAR;
 
 // Placement of the lane-private working arrays
 // (preprocessor tests in the kernels use the RADM2_PLACE_* macros of radm2.h)
@@ -75,12 +79,15 @@ constexpr int hot_regs(int placement) { return kHotConfig[placement][1]; }
 // Kernel variants: one translation unit per (placement P, register budget
 // W = minimum resident waves per SIMD (1, 2, 4, 8), launder interval R of the
 // compiler-scheduled kernels, 0 in every built variant); block sizes 64 and
-// 256 in each.
-#define RADM2_DECLARE_VARIANT(P, W, R)                                                   \
-  hipError_t launch_grid_p##P##_w##W##_r##R(int block, const LaunchArgs& a, hipStream_t st); \
-  hipError_t launch_persistent_p##P##_w##W##_r##R(int block, int blocks, const LaunchArgs& a, \
-                                                  hipStream_t st);                      \
-  int persistent_blocks_per_cu_p##P##_w##W##_r##R(int block);
+// 256 in each.  The Ros3 variants compile Ros3 alone; the family variant
+// (suffix _m1) runs every method of KPP's Rosenbrock family from one table
+// of coefficients, at the cost of registers and stage storage that would
+// slow Ros3.
+#define RADM2_DECLARE_VARIANT_NAMED(S)                                                   \
+  hipError_t launch_grid##S(int block, const LaunchArgs& a, hipStream_t st);             \
+  hipError_t launch_persistent##S(int block, int blocks, const LaunchArgs& a, hipStream_t st); \
+  int persistent_blocks_per_cu##S(int block);
+#define RADM2_DECLARE_VARIANT(P, W, R) RADM2_DECLARE_VARIANT_NAMED(_p##P##_w##W##_r##R)
 RADM2_DECLARE_VARIANT(0, 1, 0)
 RADM2_DECLARE_VARIANT(0, 2, 0)
 RADM2_DECLARE_VARIANT(0, 4, 0)
@@ -91,17 +98,21 @@ RADM2_DECLARE_VARIANT(1, 4, 0)
 RADM2_DECLARE_VARIANT(2, 1, 0)
 RADM2_DECLARE_VARIANT(3, 1, 0)
 RADM2_DECLARE_VARIANT(4, 1, 0)
+RADM2_DECLARE_VARIANT_NAMED(_p4_w1_r0_m1)
 #undef RADM2_DECLARE_VARIANT
+#undef RADM2_DECLARE_VARIANT_NAMED
 
 struct KernelVariant {
   int placement;
   int waves;
   int launder_rows;
+  int family;      // 1: KPP's five methods; 0: Ros3 only
+  int ws_doubles;  // workspace doubles per lane
   hipError_t (*grid)(int, const LaunchArgs&, hipStream_t);
   hipError_t (*persistent)(int, int, const LaunchArgs&, hipStream_t);
   int (*blocks_per_cu)(int);
 };
-constexpr int kNumVariants = 10;
+constexpr int kNumVariants = 11;
 const KernelVariant* kernel_variants();
 
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);

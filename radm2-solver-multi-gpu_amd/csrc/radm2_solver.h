@@ -44,6 +44,52 @@ constexpr double ROS_GAMMA1 = 0.43586652150845899941601945119356;
 constexpr double ROS_ELO = 3.0;
 constexpr double DELTA_MIN = 1.0e-6;
 
+// KPP's Rosenbrock family by IPAR[3] number: 1 Ros2, 2 Ros3, 3 Ros4,
+// 4 Rodas3, 5 Rodas4 -- coefficients as published (Sandu, Verwer, Blom,
+// Spee, Carmichael and Potra 1997, Atmos. Environ. 31; Hairer and Wanner
+// 1996, RODAS).  A and C hold the strictly lower triangles packed by rows,
+// entry (i, j), j < i, at i*(i-1)/2 + j.  The reference initialises only
+// Ros3 (mgpu.cu:1008-1062, 3802-3811).
+struct RosMethod {
+  int S;
+  double A[15], C[15], M[6], E[6];
+  int newf[6];
+  double gamma1, elo;
+};
+constexpr double ROS2_G = 1.0 + 1.0 / 1.4142135623730951;  // 1 + 1/sqrt(2)
+__device__ constexpr RosMethod kRosMethods[6] = {
+    {},
+    {2, {1.0 / ROS2_G}, {-2.0 / ROS2_G}, {3.0 / (2.0 * ROS2_G), 1.0 / (2.0 * ROS2_G)},
+     {1.0 / (2.0 * ROS2_G), 1.0 / (2.0 * ROS2_G)}, {1, 1}, ROS2_G, 2.0},
+    {3, {1.0, 1.0, 0.0}, {ROS_C21, ROS_C31, ROS_C32}, {ROS_M1, ROS_M2, ROS_M3},
+     {ROS_E1, ROS_E2, ROS_E3}, {1, 1, 0}, ROS_GAMMA1, ROS_ELO},
+    {4,
+     {2.0, 1.867943637803922, 0.2344449711399156, 1.867943637803922, 0.2344449711399156, 0.0},
+     {-7.137615036412310, 2.580708087951457, 0.6515950076447975, -2.137148994382534,
+      -0.3214669691237626, -0.6949742501781779},
+     {2.255570073418735, 0.2870493262186792, 0.4353179431840180, 1.093502252409163},
+     {-0.2815431932141155, -0.07276199124938920, -0.1082196201495311, -1.093502252409163},
+     {1, 1, 1, 0},
+     0.5728200000000000,
+     4.0},
+    {4, {0.0, 2.0, 0.0, 2.0, 0.0, 1.0}, {4.0, 1.0, -1.0, 1.0, -1.0, -(8.0 / 3.0)},
+     {2.0, 0.0, 1.0, 1.0}, {0.0, 0.0, 0.0, 1.0}, {1, 0, 1, 1}, 0.5, 3.0},
+    {6,
+     {1.544000000000000, 0.9466785280815826, 0.2557011698983284, 3.314825187068521,
+      2.896124015972201, 0.9986419139977817, 1.221224509226641, 6.019134481288629,
+      12.53708332932087, -0.6878860361058950, 1.221224509226641, 6.019134481288629,
+      12.53708332932087, -0.6878860361058950, 1.0},
+     {-5.668800000000000, -2.430093356833875, -0.2063599157091915, -0.1073529058151375,
+      -9.594562251023355, -20.47028614809616, 7.496443313967647, -10.24680431464352,
+      -33.99990352819905, 11.70890893206160, 8.083246795921522, -7.981132988064893,
+      -31.52159432874371, 16.31930543123136, -6.058818238834054},
+     {1.221224509226641, 6.019134481288629, 12.53708332932087, -0.6878860361058950, 1.0, 1.0},
+     {0.0, 0.0, 0.0, 0.0, 0.0, 1.0},
+     {1, 1, 1, 1, 1, 1},
+     0.25,
+     4.0},
+};
+
 // Per-cell status codes (KPP IERR)
 constexpr int ST_RUNNING = 0;
 constexpr int ST_OK = 1;
@@ -56,7 +102,7 @@ struct Ros3Opts {
   double dt, rtol, atol;
   double hmin, hmax, hstart, facmin, facmax, facrej, facsafe, roundoff;
   int32_t max_steps;
-  int32_t pad;
+  int32_t method;  // KPP's IPAR[3] number: 1 Ros2, 2 Ros3, 3 Ros4, 4 Rodas3, 5 Rodas4
 };
 
 __device__ __forceinline__ double dmax_ref(double a, double b) { return a >= b ? a : b; }
@@ -182,12 +228,75 @@ __device__ __forceinline__ void vmap(OA out, Fn fn, const IA&... in) {
   }
 }
 
-// One Ros3 attempt for a running cell whose attempt top has been applied.
-//   y   state (updated on acceptance)     k   rate constants
-//   g   659-entry LU workspace            f, k1, k2, k3: 59-entry vectors
-// R: launder interval of the compiler-scheduled fused Jacobian/decomposition
-// (rows) and, as 16 reactions per unit, of the tendency on private arrays;
-// phases are laundered apart whenever the arrays are workspace views.
+// Stages of one attempt of KPP's generic Rosenbrock scheme on workspace
+// views (the methods other than Ros3): stage s's vector lives in the
+// workspace region k1.region + s*NVAR.  The reference's commented stage
+// loop (mgpu.cu:3595-3625) in KPP's general form: F = f(y) for stage 0,
+// f(y + sum_j A(s,j) K_j) when NewF(s), else the previous F; K_s =
+// G^-1 (F + sum_j C(s,j)/H K_j).  Returns the sum of squared scaled errors
+// (sum_j E_j K_j) and leaves Ynew = y + sum_j M_j K_j in f.  The loops over
+// species are unrolled (constant element indices keep the hot set in LDS
+// and registers); the loops over stages run at run time.
+template <class KA, class YA, class GA, class VA>
+__device__ __forceinline__ double generic_stages(const RosMethod& rm, KA k, YA y, double h2o,
+                                                 double m, GA g, VA f, VA k1, double H,
+                                                 double rtol, double atol) {
+  const int S = rm.S;
+#pragma nounroll
+  for (int st = 0; st < S; st++) {
+    const int base = st * (st - 1) / 2;
+    VA ks = k1;
+    ks.region = k1.region + st * NVAR;
+    if (st == 0) {
+      gen::fun_pf16(k, y, h2o, m, f);
+    } else if (rm.newf[st]) {
+#pragma unroll
+      for (int i = 0; i < NVAR; i++) {
+        double v = RD(y, i);
+        for (int j = 0; j < st; j++) {
+          VA kj = k1;
+          kj.region = k1.region + j * NVAR;
+          v = fma(rm.A[base + j], RD(kj, i), v);
+        }
+        WR(ks, i, v);
+      }
+      gen::fun_pf16(k, ks, h2o, m, f);
+    }
+#pragma unroll
+    for (int i = 0; i < NVAR; i++) {
+      double v = RD(f, i);
+      for (int j = 0; j < st; j++) {
+        VA kj = k1;
+        kj.region = k1.region + j * NVAR;
+        v = fma(RD(kj, i), rm.C[base + j] / H, v);
+      }
+      WR(ks, i, v);
+    }
+    launder(ks);
+    gen::solve_pf16(g, ks);
+  }
+  double err = 0.0;
+#pragma unroll
+  for (int i = 0; i < NVAR; i++) {
+    const double a0 = RD(k1, i);
+    double sm = rm.M[0] * a0, e = rm.E[0] * a0;
+    for (int j = 1; j < S; j++) {
+      VA kj = k1;
+      kj.region = k1.region + j * NVAR;
+      const double aj = RD(kj, i);
+      sm = fma(rm.M[j], aj, sm);
+      e = fma(rm.E[j], aj, e);
+    }
+    const double yi = RD(y, i);
+    const double yn = yi + sm;
+    const double ymax = dmax_ref(fabs(yi), fabs(yn));
+    const double q = e / fma(rtol, ymax, atol);
+    err = fma(q, q, err);
+    WR(f, i, yn);
+  }
+  return err;
+}
+
 // fn(i, {in0[i], in1[i], ..}) for i = 0, 1, .., 58 in order, the inputs
 // streamed CH at a time with the next chunk's loads issued before this
 // chunk's work (fenced, so the compiler keeps them apart); fn may store to
@@ -223,13 +332,25 @@ __device__ __forceinline__ void xpass(Fn fn, const IA&... in) {
   }
 }
 
-template <int R, class KA, class YA, class GA, class VA>
+// One attempt for a running cell whose attempt top has been applied.
+//   y   state (updated on acceptance)     k   rate constants
+//   g   659-entry LU workspace            f, k1, k2, k3: 59-entry vectors
+// R: launder interval of the compiler-scheduled fused Jacobian/decomposition
+// (rows) and, as 16 reactions per unit, of the tendency on private arrays;
+// phases are laundered apart whenever the arrays are workspace views.
+// FAMILY: the kernel runs every method of KPP's Rosenbrock family (workspace
+// views with six stage regions); otherwise Ros3 alone, whatever o.method.
+template <int R, bool FAMILY, class KA, class YA, class GA, class VA>
 __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA k, YA y,
                                              double h2o, double m, GA g, VA f, VA k1, VA k2,
                                              VA k3) {
+  static_assert(!FAMILY || is_ws_like<GA>::value, "the family needs workspace stage regions");
   constexpr int RF = R > 0 ? 16 * R : 0;
   const double H = s.H;
-  const double ghinv = 1.0 / (H * ROS_GAMMA1);
+  const bool ros3 = !FAMILY || o.method == 2;
+  const RosMethod& rm = kRosMethods[ros3 ? 2 : o.method];
+  const double elo = ros3 ? ROS_ELO : rm.elo;
+  const double ghinv = 1.0 / (H * (ros3 ? ROS_GAMMA1 : rm.gamma1));
   const double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
   const double rtol = o.rtol, atol = o.atol;
   double err = 0.0;
@@ -243,6 +364,9 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
       s.status = ST_SINGULAR;
       return;
     }
+    if (!ros3) {
+      err = generic_stages(rm, k, y, h2o, m, g, f, k1, H, rtol, atol);
+    } else {
     double x[NVAR];
     // K1 = G^-1 f(y)
     gen::fun_pf16_reg(k, y, h2o, m, x);
@@ -274,6 +398,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
       err = fma(q, q, err);
       WR(f, i, yn);
     }, k1, k2, y);
+    }
   } else {
     // F0 = f(y); K1 = G^-1 F0
     stage_fun<RF>(k, y, h2o, m, k1);
@@ -306,7 +431,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
     }, k1, k2, k3, y);
   }
   err = sqrt(err / (double)NVAR);
-  const double fsafe = dmax_ref(o.facmin, o.facsafe / pow(err, 1.0 / ROS_ELO));
+  const double fsafe = dmax_ref(o.facmin, o.facsafe / pow(err, 1.0 / elo));
   const double fac = dmin_ref(o.facmax, fsafe);
   double Hnew = H * fac;
   s.ns++;

@@ -30,7 +30,7 @@ class OptsOut(ctypes.Structure):
                 ("hstart", ctypes.c_double), ("facmin", ctypes.c_double),
                 ("facmax", ctypes.c_double), ("facrej", ctypes.c_double),
                 ("facsafe", ctypes.c_double), ("roundoff", ctypes.c_double),
-                ("max_steps", ctypes.c_int32)]
+                ("max_steps", ctypes.c_int32), ("method", ctypes.c_int32)]
 
 
 _lib = None
@@ -61,6 +61,8 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, ctypes.POINTER(OptsOut),
                                        ctypes.c_double, _I, _I, _I, _D]
         L.oracle_ros3_cell.restype = ctypes.c_int
+        L.oracle_ros_cell.argtypes = [ctypes.c_int] + L.oracle_ros3_cell.argtypes
+        L.oracle_ros_cell.restype = ctypes.c_int
         L.oracle_integrate.argtypes = [ctypes.c_int64, ctypes.c_int64, _D, _D, _D, _D, _D, _D,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.POINTER(OptsIn), _D, _I, _I, _I, _I, _D,

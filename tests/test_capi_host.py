@@ -70,8 +70,8 @@ def _expected(o, rtol, atol):
         return -4
     if atol <= 0 or rtol <= 10 * 2.0 ** -52 or rtol >= 1:
         return -5
-    if m not in (0, 2):
-        return -2
+    # KPP's five methods (the reference initialises only Ros3; the others
+    # are implemented here)
     return 0
 
 

@@ -62,3 +62,31 @@ def test_parity_tight_tolerance_reference_run(solver):
     same, ok, rel = _compare(cg, og, co, oo, "rtol 1e-10")
     # both are converged solutions: they agree far below the rtol-1e-3 error
     assert rel.max() <= 1e-6
+
+
+@pytest.mark.parametrize("method", [1, 3, 4, 5])
+def test_other_kpp_methods_match_oracle(solver, method):
+    # KPP's Ros2, Ros4, Rodas3 and Rodas4 (IPAR[3] 1, 3, 4, 5) on the family
+    # kernel (the default placement), persistent and grid schedules; the
+    # other placements are built for Ros3 alone and refuse them
+    import radm2_amd
+    from radm2_amd import synth
+    cells = synth.make_cells(256, 30 + method, 100.0, 0.05)
+    oo_in = O.OptsIn(method=method)
+    co, oo, code = O.integrate(cells, 600.0, 1e-3, 1.0, opts=oo_in, nthreads=0)
+    assert code == 0 and (oo["status"] == 1).all()
+    try:
+        for sched in (radm2_amd.SCHED_PERSISTENT, radm2_amd.SCHED_GRID):
+            cg, og = _run_gpu(solver, cells, 600.0, 1e-3, 1.0, sched, opts=radm2_amd.Opts(method=method))
+            assert (og["status"] == 1).all()
+            same, ok, rel = _compare(cg, og, co, oo, "method %d schedule %d" % (method, sched))
+            assert ok.mean() >= 0.99 and same.mean() >= 0.99
+        for pl, w in ((radm2_amd.PLACE_PRIVATE, 4), (radm2_amd.PLACE_WORKSPACE, 1),
+                      (radm2_amd.PLACE_LDS160, 1)):
+            solver.set_kernel(pl, w)
+            with pytest.raises(radm2_amd.Radm2Error) as e:
+                _run_gpu(solver, cells, 600.0, 1e-3, 1.0, 1, opts=radm2_amd.Opts(method=method))
+            assert e.value.code == radm2_amd.E_UNSUPPORTED
+    finally:
+        solver.set_kernel(-1, 0)
+        solver.set_schedule(radm2_amd.SCHED_PERSISTENT, 0, 0)

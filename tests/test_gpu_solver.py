@@ -238,7 +238,7 @@ def test_call_level_errors(solver, cells):
     sub = {k: np.ascontiguousarray(v[..., :8]) for k, v in cells.items()
            if k in ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")}
     b = _batch(sub)
-    for opts, code in ((radm2_amd.Opts(max_steps=-1), -1), (radm2_amd.Opts(method=3), -2),
+    for opts, code in ((radm2_amd.Opts(max_steps=-1), -1), (radm2_amd.Opts(method=6), -2),
                        (radm2_amd.Opts(hmin=-1.0), -3), (radm2_amd.Opts(facmax=-1.0), -4)):
         with pytest.raises(radm2_amd.Radm2Error) as e:
             solver.integrate(b, 600.0, 1e-3, 1.0, opts=opts)
@@ -273,6 +273,18 @@ def test_kpp_frontend_statistics(solver, cells):
     IPAR2 = [0] * 20
     IPAR2[0] = 1
     assert solver.rosenbrock(b, 0.0, 240.0, [1.0] * 59, [1e-3] * 59, [0.0] * 20, IPAR2) == -2
+    # Rodas4 (IPAR[3] = 5): six solves per attempt, five new F evaluations
+    b5 = _batch(sub)
+    RPAR5 = [0.0] * 20
+    IPAR5 = [0] * 20
+    IPAR5[0] = 1
+    IPAR5[3] = 5
+    RPAR5[2] = 0.01 * 240.0
+    assert solver.rosenbrock(b5, 0.0, 240.0, [1.0] * 59, [1e-3] * 59, RPAR5, IPAR5) == 1
+    co5, oo5, code5 = O.integrate(sub, 240.0, 1e-3, 1.0, opts=O.OptsIn(hstart=2.4, method=5))
+    assert code5 == 0
+    assert abs(IPAR5[12] - oo5["nstep"].sum()) <= 0.01 * oo5["nstep"].sum()
+    assert IPAR5[16] == 6 * IPAR5[12] and IPAR5[10] == IPAR5[13] + 5 * IPAR5[12]
 
 
 def test_batched_64_uneven_domains_plan_matches_oracle(solver):
