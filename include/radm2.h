@@ -62,13 +62,18 @@ extern "C" {
  * reference's Rosenbrock() (mgpu.cu:3698-3791): a zero field selects KPP's
  * default (Hmin 0, Hmax dt, Hstart max(Hmin,1e-6), FacMin 0.2, FacMax 6,
  * FacRej 0.1, FacSafe 0.9, 100000 steps); negative values are errors.
- * method: 0 or 2 = Ros3, the only method the reference implements.
+ * method: KPP's IPAR[3] number, 0 or 2 = Ros3 (the only method the
+ * reference initialises), 1 Ros2, 3 Ros4, 4 Rodas3, 5 Rodas4.
+ * sing_retries: step halvings after a singular matrix before -8 (KPP's
+ * ros_PrepareMatrix allows 5; 0 = the reference's active code, mgpu.cu:
+ * 3147-3163, which fails at once); negative values are RADM2_E_ARG.
  * Passing NULL instead selects INTEGRATE()'s call (mgpu.cu:3866-3873):
  * all defaults except Hstart = 0.01*dt.
  */
 typedef struct radm2_opts {
   double hmin, hmax, hstart, facmin, facmax, facrej, facsafe;
   int32_t max_steps, method;
+  int32_t sing_retries;
 } radm2_opts;
 
 typedef struct radm2_ctx radm2_ctx;
@@ -89,6 +94,11 @@ typedef struct radm2_domain {
   double *hfirst;         /* [ld] controller proposal after the first accepted step (nullable) */
   double dt, rtol, atol;
   const radm2_opts *opts; /* NULL = INTEGRATE() defaults */
+  /* [21][ld] photolysis rates at the end of the interval (s^-1), or NULL.
+   * Non-NULL makes the integration non-autonomous (KPP's IPAR[0] = 0): the
+   * photolysis rate constants follow J(t) = jrate + (t/dt)(jrate_end -
+   * jrate), dF/dT enters every stage (mgpu.cu:1566-1593, 3619-3622). */
+  const double *jrate_end;
 } radm2_domain;
 
 /* Create a context on HIP device `device`.  stream: a hipStream_t to run on,
@@ -164,15 +174,19 @@ int radm2_last_elapsed_ms(radm2_ctx *ctx, float *ms);
 int radm2_enable_stats(radm2_ctx *ctx, int on);
 int radm2_read_stats(radm2_ctx *ctx, uint64_t out[4]);
 
-/* KPP option check without running: returns 0 or the IERR code -1..-5 of
- * Rosenbrock() (mgpu.cu:3715-3811). */
+/* KPP option check without running: returns 0, the IERR code -1..-5 of
+ * Rosenbrock() (mgpu.cu:3715-3811), or RADM2_E_ARG for a negative
+ * sing_retries. */
 int radm2_check_opts(double dt, double rtol, double atol, const radm2_opts *opts);
 
 /*
  * KPP front-end with the reference's own calling convention: Rosenbrock(
  * Tstart, Tend, AbsTol, RelTol, RPAR, IPAR) (mgpu.cu:3664-3848) over the
  * cells of `state` (its dt/rtol/atol/opts fields are ignored).  IPAR[0]
- * must be 1 (autonomous; the reference's only use), IPAR[1] selects scalar
+ * nonzero: autonomous (the reference's only use, mgpu.cu:3871); 0:
+ * non-autonomous over state->jrate_end (NULL: J constant, so dF/dT = 0 and
+ * the result is the autonomous one; Nfun counts the time derivative's
+ * evaluation either way).  IPAR[1] selects scalar
  * (1) or vector (0) tolerances (vectors must hold one value for all 59
  * species), IPAR[2..3] and RPAR[0..6] as radm2_opts.  On return IPAR[10..17]
  * = Nfun, Njac, Nstp, Nacc, Nrej, Ndec, Nsol, Nsng summed over cells, and

@@ -55,6 +55,12 @@ def main(dst=None):
     L.append("static const int O_DROP_I[%d] = {%s};" % (len(d), ",".join(str(x[0]) for x in d)))
     L.append("static const int O_DROP_R[%d] = {%s};" % (len(d), ",".join(str(x[1]) for x in d)))
     L.append("static const double O_DROP_C[%d] = {%s};" % (len(d), ",".join(x[2] + ".0" for x in d)))
+    # photolysis: reactions whose rate law is a bare J[s]
+    pj = {r: int(re.fullmatch(r"\(J\[(\d+)\]\)", e.strip()).group(1))
+          for r, e in enumerate(mech["rates"]) if re.fullmatch(r"\(J\[(\d+)\]\)", e.strip())}
+    L.append("#define O_NPHOTO %d" % len(pj))
+    L.append("static const int O_PHOTO_R[%d] = {%s};" % (len(pj), ",".join(map(str, pj))))
+    L.append("static const int O_PHOTO_J[%d] = {%s};" % (len(pj), ",".join(map(str, pj.values()))))
     lu = mech["lu"]
     L.append("static const int O_LU_CROW[60] = {%s};" % ",".join(map(str, lu["crow"])))
     L.append("static const int O_LU_DIAG[59] = {%s};" % ",".join(map(str, lu["diag"])))

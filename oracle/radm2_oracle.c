@@ -238,6 +238,8 @@ int oracle_resolve(double dt, double rtol, double atol, const oracle_opts_in *in
   /* the reference initialises only Ros3 (mgpu.cu:3802-3811); KPP's other
    * four methods are implemented here */
   o->method = method;
+  if (in->sing_retries < 0) return O_E_ARG;
+  o->sing_retries = in->sing_retries;
   return 0;
 }
 
@@ -328,10 +330,19 @@ done:
  * mgpu.cu:3595-3625, in KPP's general form):
  *   F = f(y) for stage 0; F = f(y + sum_j A(s,j) K_j) when NewF(s), else
  *   the previous stage's F; K_s = G^-1 (F + sum_j C(s,j)/H K_j);
- *   Ynew = y + sum M_j K_j, error sum E_j K_j.  Autonomous only. */
+ *   Ynew = y + sum M_j K_j, error sum E_j K_j.
+ * Non-autonomous (KPP, mgpu.cu:1566-1593 and 3619-3622): dF/dT by a
+ * difference quotient (F(T + Delta) - F(T)) / Delta once per time step, a new
+ * F of stage s evaluated at T + alpha_s H, and H gamma_s dF/dT added to stage
+ * s's right-hand side.  alpha_i = sum_j alpha_ij and gamma_i = sum_j
+ * gamma_ij of the published methods.  The forcing is linear in t, so any
+ * Delta gives the exact slope; Delta reaches to the farther end of the
+ * interval (KPP's sqrt(roundoff) max(DeltaMin, |T|) would cancel to noise
+ * near T = 0, where every call's time axis starts).  The reference's own
+ * time derivative evaluates Fun at T itself (its dF/dT is zero). */
 typedef struct {
   int S;
-  double A[15], C[15], M[6], E[6], gamma1, elo;
+  double A[15], C[15], M[6], E[6], alpha[6], gamma[6], gamma1, elo;
   int newf[6];
 } o_method;
 
@@ -348,6 +359,9 @@ static o_method o_method_table(int method) {
     r.M[1] = 1.0 / (2.0 * g);
     r.E[0] = 1.0 / (2.0 * g);
     r.E[1] = 1.0 / (2.0 * g);
+    r.alpha[1] = 1.0;
+    r.gamma[0] = g;
+    r.gamma[1] = -g;
     r.gamma1 = g;
     r.elo = 2.0;
   } else if (method == 2) { /* Ros3 */
@@ -357,6 +371,11 @@ static o_method o_method_table(int method) {
     r.newf[0] = 1; r.newf[1] = 1; r.newf[2] = 0;
     r.M[0] = ROS_M1; r.M[1] = ROS_M2; r.M[2] = ROS_M3;
     r.E[0] = ROS_E1; r.E[1] = ROS_E2; r.E[2] = ROS_E3;
+    /* mgpu.cu:1054-1061 */
+    r.alpha[1] = r.alpha[2] = 0.43586652150845899941601945119356;
+    r.gamma[0] = 0.43586652150845899941601945119356;
+    r.gamma[1] = 0.24291996454816804366592249683314;
+    r.gamma[2] = 2.1851380027664058511513169485832;
     r.gamma1 = ROS_GAMMA1;
     r.elo = ROS_ELO;
   } else if (method == 3) { /* Ros4 */
@@ -370,6 +389,12 @@ static o_method o_method_table(int method) {
     r.M[2] = 0.4353179431840180; r.M[3] = 1.093502252409163;
     r.E[0] = -0.2815431932141155; r.E[1] = -0.07276199124938920;
     r.E[2] = -0.1082196201495311; r.E[3] = -1.093502252409163;
+    r.alpha[1] = 1.145640000000000;
+    r.alpha[2] = r.alpha[3] = 0.6552168638155900;
+    r.gamma[0] = 0.5728200000000000;
+    r.gamma[1] = -1.769193891319233;
+    r.gamma[2] = 0.7592633437920482;
+    r.gamma[3] = -0.1049021087100450;
     r.gamma1 = 0.5728200000000000;
     r.elo = 4.0;
   } else if (method == 4) { /* Rodas3 */
@@ -379,6 +404,9 @@ static o_method o_method_table(int method) {
     r.newf[0] = 1; r.newf[1] = 0; r.newf[2] = 1; r.newf[3] = 1;
     r.M[0] = 2.0; r.M[1] = 0.0; r.M[2] = 1.0; r.M[3] = 1.0;
     r.E[3] = 1.0;
+    r.alpha[2] = r.alpha[3] = 1.0;
+    r.gamma[0] = 0.5;
+    r.gamma[1] = 1.5;
     r.gamma1 = 0.5;
     r.elo = 3.0;
   } else { /* Rodas4 */
@@ -398,6 +426,14 @@ static o_method o_method_table(int method) {
     r.M[0] = a[6]; r.M[1] = a[7]; r.M[2] = a[8]; r.M[3] = a[9]; r.M[4] = 1.0; r.M[5] = 1.0;
     r.E[5] = 1.0;
     for (int i = 0; i < 6; i++) r.newf[i] = 1;
+    r.alpha[1] = 0.386;
+    r.alpha[2] = 0.210;
+    r.alpha[3] = 0.630;
+    r.alpha[4] = r.alpha[5] = 1.0;
+    r.gamma[0] = 0.25;
+    r.gamma[1] = -0.1043;
+    r.gamma[2] = 0.1035;
+    r.gamma[3] = -0.3620000000000023e-01;
     r.gamma1 = 0.25;
     r.elo = 4.0;
   }
@@ -406,11 +442,41 @@ static o_method o_method_table(int method) {
 
 int oracle_method_stages(int method) { return o_method_table(method).S; }
 
-int oracle_ros_cell(int method, double *y, const double *k, double h2o, double m, double dt,
-                    double rtol, double atol, const oracle_opts *o, double h0, int32_t *nstep,
-                    int32_t *nacc, int32_t *nrej, double *hexit) {
+void oracle_method_coeffs(int method, double *A, double *C, double *M, double *E,
+                          double *alpha, double *gamma, double *g_elo) {
   const o_method r = o_method_table(method);
-  double G[O_LU_NNZ], F0[O_NVAR], F[O_NVAR], K[6][O_NVAR], yn[O_NVAR];
+  memcpy(A, r.A, sizeof r.A);
+  memcpy(C, r.C, sizeof r.C);
+  memcpy(M, r.M, sizeof r.M);
+  memcpy(E, r.E, sizeof r.E);
+  memcpy(alpha, r.alpha, sizeof r.alpha);
+  memcpy(gamma, r.gamma, sizeof r.gamma);
+  g_elo[0] = r.gamma1;
+  g_elo[1] = r.elo;
+}
+
+/* time step of the dF/dT difference quotient: to the farther end of [0, dt] */
+static double o_forcing_delta(double T, double dt) { return T <= 0.5 * dt ? dt - T : -T; }
+
+/* J(t) = j0 + (t/dt)(j1 - j0) for the photolysis reactions (rate constant
+ * = the bare J, mgpu.cu:763-783) */
+void oracle_photo_at(double *k, const double *j0, const double *j1, double t, double dt) {
+  const double w = t / dt;
+  for (int q = 0; q < O_NPHOTO; q++) {
+    const int s = O_PHOTO_J[q];
+    k[O_PHOTO_R[q]] = fma(w, j1[s] - j0[s], j0[s]);
+  }
+}
+
+int oracle_ros_cell(int method, double *y, const double *k, const double *j0, const double *j1,
+                    double h2o, double m, double dt, double rtol, double atol,
+                    const oracle_opts *o, double h0, int32_t *nstep, int32_t *nacc,
+                    int32_t *nrej, double *hexit) {
+  const o_method r = o_method_table(method);
+  double G[O_LU_NNZ], F0[O_NVAR], F[O_NVAR], K[6][O_NVAR], yn[O_NVAR], dFdT[O_NVAR];
+  double kk[O_NREACT];
+  const int nonauto = j1 != NULL;
+  memcpy(kk, k, sizeof kk);
   const double Tend = dt, rnd = o->roundoff;
   double T = 0.0;
   double H = O_MIN(o->hstart, o->hmax);
@@ -423,14 +489,29 @@ int oracle_ros_cell(int method, double *y, const double *k, double h2o, double m
   int32_t ns = 0, na = 0, nr = 0;
   while ((T - Tend) + rnd <= 0.0) {
     int first = 1;
-    oracle_fun(y, h2o, m, k, F0);
+    if (nonauto) oracle_photo_at(kk, j0, j1, T, dt);
+    oracle_fun(y, h2o, m, kk, F0);
+    if (nonauto) { /* ros_FunTimeDerivative with T perturbed */
+      const double delta = o_forcing_delta(T, dt);
+      oracle_photo_at(kk, j0, j1, T + delta, dt);
+      oracle_fun(y, h2o, m, kk, dFdT);
+      for (int i = 0; i < O_NVAR; i++) dFdT[i] = fma(-1.0, F0[i], dFdT[i]) * (1.0 / delta);
+    }
     for (;;) {
       if (ns > o->max_steps) { status = -6; goto done; }
       if (T + 0.1 * H == T || !(H > rnd)) { status = -7; goto done; }
       if (first) { H = O_MIN(H, fabs(Tend - T)); first = 0; }
-      const double ghinv = 1.0 / (H * r.gamma1);
-      oracle_jac_prep(y, h2o, m, k, ghinv, G);
-      if (oracle_decomp(G, 0) != 0) { status = -8; goto done; }
+      if (nonauto) oracle_photo_at(kk, j0, j1, T, dt); /* the Jacobian at T */
+      /* ros_PrepareMatrix (KPP): a singular matrix halves H and retries, up
+       * to sing_retries times (the reference's active code: none,
+       * mgpu.cu:3147-3163), then -8 */
+      for (int nsc = 0;; nsc++) {
+        const double ghinv = 1.0 / (H * r.gamma1);
+        oracle_jac_prep(y, h2o, m, kk, ghinv, G);
+        if (oracle_decomp(G, 0) == 0) break;
+        if (nsc >= o->sing_retries) { status = -8; goto done; }
+        H = H * 0.5;
+      }
       for (int st = 0; st < r.S; st++) {
         const int base = st * (st - 1) / 2;
         if (st == 0) {
@@ -441,12 +522,17 @@ int oracle_ros_cell(int method, double *y, const double *k, double h2o, double m
             for (int j = 0; j < st; j++) v = fma(r.A[base + j], K[j][i], v);
             yn[i] = v;
           }
-          oracle_fun(yn, h2o, m, k, F);
+          if (nonauto) oracle_photo_at(kk, j0, j1, T + r.alpha[st] * H, dt);
+          oracle_fun(yn, h2o, m, kk, F);
         }
         for (int i = 0; i < O_NVAR; i++) {
           double v = F[i];
           for (int j = 0; j < st; j++) v = fma(K[j][i], r.C[base + j] / H, v);
           K[st][i] = v;
+        }
+        if (nonauto && r.gamma[st] != 0.0) {
+          const double hg = H * r.gamma[st];
+          for (int i = 0; i < O_NVAR; i++) K[st][i] = fma(hg, dFdT[i], K[st][i]);
         }
         oracle_solve(G, K[st], 0);
       }
@@ -493,7 +579,8 @@ done:
 
 int oracle_integrate(int64_t n, int64_t ld, double *conc, const double *temp,
                      const double *cair, const double *h2o, const double *jrate,
-                     const double *rc_n2o5, double dt, double rtol, double atol,
+                     const double *jrate_end, const double *rc_n2o5, double dt, double rtol,
+                     double atol,
                      const oracle_opts_in *opts, const double *h0, int32_t *nstep,
                      int32_t *nacc, int32_t *nrej, int32_t *status, double *hexit,
                      int nthreads) {
@@ -506,16 +593,20 @@ int oracle_integrate(int64_t n, int64_t ld, double *conc, const double *temp,
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads)
 #endif
   for (int64_t p = 0; p < n; p++) {
-    double y[O_NVAR], k[O_NREACT], J[O_NJ];
+    double y[O_NVAR], k[O_NREACT], J[O_NJ], J1[O_NJ];
     for (int j = 0; j < O_NJ; j++) J[j] = jrate[j * ld + p];
+    if (jrate_end)
+      for (int j = 0; j < O_NJ; j++) J1[j] = jrate_end[j * ld + p];
     oracle_rconst(temp[p], cair[p], J, rc_n2o5 ? rc_n2o5[p] : rc_def, k);
     for (int i = 0; i < O_NVAR; i++) y[i] = conc[i * ld + p];
     int32_t a = 0, b = 0, c = 0;
     double hx = 0.0;
-    int st = o.method != 2 ? oracle_ros_cell(o.method, y, k, h2o[p], cair[p], dt, rtol, atol, &o,
-                                             h0 ? h0[p] : 0.0, &a, &b, &c, &hx)
-                           : oracle_ros3_cell(y, k, h2o[p], cair[p], dt, rtol, atol, &o, h0 ? h0[p] : 0.0, &a,
-                              &b, &c, &hx);
+    const double h0p = h0 ? h0[p] : 0.0;
+    int st = (o.method != 2 || o.sing_retries > 0 || jrate_end)
+                 ? oracle_ros_cell(o.method, y, k, J, jrate_end ? J1 : NULL, h2o[p], cair[p], dt,
+                                   rtol, atol, &o, h0p, &a, &b, &c, &hx)
+                 : oracle_ros3_cell(y, k, h2o[p], cair[p], dt, rtol, atol, &o, h0p, &a, &b, &c,
+                                    &hx);
     for (int i = 0; i < O_NVAR; i++) conc[i * ld + p] = y[i];
     if (nstep) nstep[p] = a;
     if (nacc) nacc[p] = b;

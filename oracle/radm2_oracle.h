@@ -21,15 +21,19 @@ extern "C" {
 #define O_LU_NNZ 659
 #define O_NJ 21
 
+/* sing_retries: step halvings allowed after a singular decomposition
+ * (0 = the reference's active code, -8 at once; KPP: 5) */
 typedef struct {
   double hmin, hmax, hstart, facmin, facmax, facrej, facsafe;
-  int32_t max_steps, method;
+  int32_t max_steps, method, sing_retries;
 } oracle_opts_in;
 
 typedef struct {
   double hmin, hmax, hstart, facmin, facmax, facrej, facsafe, roundoff;
-  int32_t max_steps, method; /* method: KPP number 1..5 */
+  int32_t max_steps, method, sing_retries; /* method: KPP number 1..5 */
 } oracle_opts;
+
+#define O_E_ARG (-101) /* negative sing_retries */
 
 /* building blocks, one cell */
 void oracle_rconst(double T, double M, const double *J21, double rc_n2o5, double *k);
@@ -50,10 +54,13 @@ int oracle_ros3_cell(double *y, const double *k, double h2o, double m, double dt
                      double atol, const oracle_opts *o, double h0, int32_t *nstep,
                      int32_t *nacc, int32_t *nrej, double *hexit);
 
-/* many cells, SoA; returns the call-level code (0 or -1..-5) */
+/* many cells, SoA; returns the call-level code (0, -1..-5 or O_E_ARG).
+ * jrate_end: NULL (autonomous) or the J rates at the end of the interval,
+ * J(t) linear in between (non-autonomous). */
 int oracle_integrate(int64_t n, int64_t ld, double *conc, const double *temp,
                      const double *cair, const double *h2o, const double *jrate,
-                     const double *rc_n2o5, double dt, double rtol, double atol,
+                     const double *jrate_end, const double *rc_n2o5, double dt, double rtol,
+                     double atol,
                      const oracle_opts_in *opts, const double *h0, int32_t *nstep,
                      int32_t *nacc, int32_t *nrej, int32_t *status, double *hexit,
                      int nthreads);
@@ -67,11 +74,22 @@ void oracle_wrf_convert(const int *grid, const double *phy, const double *moist,
                         double *rc_n2o5);
 
 /* KPP's Rosenbrock family by IPAR[3] number (1 Ros2, 2 Ros3, 3 Ros4,
- * 4 Rodas3, 5 Rodas4): one cell with the generic stage loop. */
-int oracle_ros_cell(int method, double *y, const double *k, double h2o, double m, double dt,
-                    double rtol, double atol, const oracle_opts *o, double h0, int32_t *nstep,
-                    int32_t *nacc, int32_t *nrej, double *hexit);
+ * 4 Rodas3, 5 Rodas4): one cell with the generic stage loop.  j1 NULL:
+ * autonomous (rate constants k throughout).  Otherwise non-autonomous: the
+ * photolysis rate constants follow J(t) = j0 + (t/dt)(j1 - j0) (21 values
+ * each, the reference's jv order), with KPP's finite-difference dF/dT. */
+int oracle_ros_cell(int method, double *y, const double *k, const double *j0, const double *j1,
+                    double h2o, double m, double dt, double rtol, double atol,
+                    const oracle_opts *o, double h0, int32_t *nstep, int32_t *nacc,
+                    int32_t *nrej, double *hexit);
 int oracle_method_stages(int method);
+/* a method's tables: A[15], C[15] (packed lower triangles), M[6], E[6],
+ * alpha[6] (stage times), gamma[6] (row sums of its gamma matrix),
+ * g_elo[2] = {gamma, ELO} */
+void oracle_method_coeffs(int method, double *A, double *C, double *M, double *E,
+                          double *alpha, double *gamma, double *g_elo);
+/* photolysis rate constants at time t of [0, dt] into k */
+void oracle_photo_at(double *k, const double *j0, const double *j1, double t, double dt);
 
 #ifdef __cplusplus
 }

@@ -139,6 +139,21 @@ def neg(s):
     return s[1:] if s.startswith("-") else "-" + s
 
 
+def photo_j(mech):
+    """J index of each photolysis reaction: the rate laws that are a bare
+    J[s] are exactly the first ones, in reaction order."""
+    import re
+    out = []
+    for e in mech["rates"]:
+        mt = re.fullmatch(r"\(J\[(\d+)\]\)", e.strip())
+        if not mt:
+            break
+        out.append(int(mt.group(1)))
+    assert all(not re.search(r"\bJ\[", e) for e in mech["rates"][len(out):])
+    assert sorted(out) == list(range(21))
+    return out
+
+
 def gen_rconst(mech):
     em = RateEmitter()
     body = []
@@ -728,6 +743,9 @@ def generate(dst=None, mech_path=None):
         "constexpr int NREACT = %d;" % mech["nreact"],
         "constexpr int LU_NNZ = %d;" % mech["lu"]["nnz"],
         "constexpr int NJ = 21;",
+        "// photolysis: reaction r < NPHOTO has rate constant J[PHOTO_J[r]]",
+        "constexpr int NPHOTO = %d;" % len(photo_j(mech)),
+        "constexpr int PHOTO_J[%d] = {%s};" % (len(photo_j(mech)), ",".join(map(str, photo_j(mech)))),
         "constexpr int DECOMP_FMA = %d;" % nfma,
         "constexpr int LU_DIAG[%d] = {%s};" % (mech["nvar"], ",".join(map(str, mech["lu"]["diag"]))),
         "",

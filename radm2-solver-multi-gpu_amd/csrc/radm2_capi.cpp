@@ -142,6 +142,9 @@ int resolve_opts(double dt, double rtol, double atol, const radm2_opts* in, Ros3
   // the reference initialises only Ros3 (mgpu.cu:3802-3811); KPP's other
   // four methods run through the generic stage loop
   o->method = method;
+  if (in->sing_retries < 0) return RADM2_E_ARG;
+  o->sing_retries = in->sing_retries;
+  o->pad = 0;
   return 0;
 }
 
@@ -174,6 +177,7 @@ int fill_desc(const radm2_domain& in, int64_t first, DomainDesc* d) {
   d->hexit = in.hexit;
   d->hfirst = in.hfirst;
   d->rc_default = default_rc_n2o5();
+  d->jrate_end = in.jrate_end;
   return 0;
 }
 
@@ -247,8 +251,9 @@ int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
 }
 
 // Device table of the domains.  *family: some domain uses a method other
-// than Ros3, which only the family variant runs (RADM2_E_UNSUPPORTED when the
-// configured placement has none).
+// than Ros3, the non-autonomous forcing or singular-matrix retries, which
+// only the family variant runs (RADM2_E_UNSUPPORTED when the configured
+// placement has none).
 int build_table(const radm2_ctx* c, const radm2_domain* doms, int ndom,
                 std::vector<DomainDesc>& out, int64_t* total, bool* family) {
   out.resize(ndom);
@@ -256,7 +261,9 @@ int build_table(const radm2_ctx* c, const radm2_domain* doms, int ndom,
   bool other_methods = false;
   for (int i = 0; i < ndom; i++) {
     if (int rc = fill_desc(doms[i], first, &out[i])) return rc;
-    other_methods |= out[i].opts.method != 2 && doms[i].ncells > 0;
+    other_methods |= (out[i].opts.method != 2 || out[i].opts.sing_retries > 0 ||
+                      doms[i].jrate_end != nullptr) &&
+                     doms[i].ncells > 0;
     first += doms[i].ncells;
   }
   *total = first;
@@ -529,7 +536,8 @@ int radm2_rosenbrock(radm2_ctx* c, const radm2_domain* state, double Tstart, dou
                      const double* AbsTol, const double* RelTol, double RPAR[20],
                      int32_t IPAR[20]) {
   if (!c || !state || !AbsTol || !RelTol || !RPAR || !IPAR) return RADM2_E_ARG;
-  if (IPAR[0] == 0) return RADM2_E_UNSUPPORTED;  // time-dependent forcing not implemented
+  // KPP: Autonomous = IPAR[0] != 0; non-autonomous runs over state->jrate_end
+  const bool autonomous = IPAR[0] != 0;
   const int ntol = IPAR[1] == 0 ? RADM2_NVAR : 1;
   for (int i = 1; i < ntol; i++)
     if (AbsTol[i] != AbsTol[0] || RelTol[i] != RelTol[0]) return RADM2_E_UNSUPPORTED;
@@ -544,6 +552,7 @@ int radm2_rosenbrock(radm2_ctx* c, const radm2_domain* state, double Tstart, dou
   o.facsafe = RPAR[6];
   o.max_steps = IPAR[2];
   o.method = IPAR[3];
+  o.sing_retries = 0;  // the reference's active code (mgpu.cu:3147-3163)
   // the reference rejects IPAR[3] outside 1..5 before defaulting (mgpu.cu:3721-3728)
   if (IPAR[3] < 1 || IPAR[3] > 5) return -2;
   const double dt = Tend - Tstart;
@@ -569,6 +578,7 @@ int radm2_rosenbrock(radm2_ctx* c, const radm2_domain* state, double Tstart, dou
   d.rtol = RelTol[0];
   d.atol = AbsTol[0];
   d.opts = &o;
+  if (autonomous) d.jrate_end = nullptr;  // constant J without the family kernel
   int rc = radm2_integrate_batched(c, 1, &d);
   std::vector<int32_t> hns(n), hna(n), hnr(n), hst(n);
   std::vector<double> hhx(n);
@@ -589,12 +599,12 @@ int radm2_rosenbrock(radm2_ctx* c, const radm2_domain* state, double Tstart, dou
       if (hst[p] != 1 && worst == 1) worst = hst[p];
       hmax_exit = std::max(hmax_exit, hhx[p]);
     }
-    // KPP accounting: one Fun+Jac per time step (accepted attempt); per
-    // attempt one Decomp, one Solve per stage and one Fun per stage after
-    // the first that evaluates a new F (Ros2 1, Ros3 1, Ros4 2, Rodas3 2,
-    // Rodas4 5)
+    // KPP accounting: one Fun+Jac per time step (accepted attempt), and one
+    // more Fun for dF/dT when non-autonomous; per attempt one Decomp, one
+    // Solve per stage and one Fun per stage after the first that evaluates a
+    // new F (Ros2 1, Ros3 1, Ros4 2, Rodas3 2, Rodas4 5)
     static const int kNewF[6] = {0, 1, 1, 2, 2, 5}, kStages[6] = {0, 2, 3, 4, 4, 6};
-    IPAR[10] += (int32_t)(nacc + kNewF[ro.method] * nstp);
+    IPAR[10] += (int32_t)((autonomous ? 1 : 2) * nacc + kNewF[ro.method] * nstp);
     IPAR[11] += (int32_t)nacc;
     IPAR[12] += (int32_t)nstp;
     IPAR[13] += (int32_t)nacc;

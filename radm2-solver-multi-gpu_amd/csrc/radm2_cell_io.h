@@ -31,6 +31,16 @@ __device__ __forceinline__ void cell_load(const DomainDesc& d, int64_t p, KA k, 
   cell_begin(s, d.opts, d.h0 ? d.h0[p] : 0.0);
 }
 
+// Photolysis forcing of cell p of domain d (j1 null when autonomous).
+__device__ __forceinline__ Forcing cell_forcing(const DomainDesc& d, int64_t p) {
+  Forcing fc;
+  fc.j0 = d.jrate + p;
+  fc.j1 = d.jrate_end ? d.jrate_end + p : nullptr;
+  fc.ld = d.ld;
+  fc.dt = d.opts.dt;
+  return fc;
+}
+
 template <class YA>
 __device__ __forceinline__ void cell_store(const DomainDesc& d, int64_t p, YA y,
                                            const CellState& s, double hfirst,

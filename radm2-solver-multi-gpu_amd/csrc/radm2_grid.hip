@@ -89,6 +89,11 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
   cell_load(d, p, kk, y, h2o, m, s);
   if (!valid) s.status = ST_OK;  // padding lane of the last block: never runs
   const Ros3Opts& o = d.opts;
+#if RADM2_METHODS
+  const Forcing fc = cell_forcing(d, p);
+#else
+  const Forcing fc{};
+#endif
   for (;;) {
     const bool run = s.status == ST_RUNNING;
     if (LOCKSTEP) {
@@ -98,7 +103,8 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
     }
     if (run) {
       cell_attempt_top(s, o);
-      if (s.status == ST_RUNNING) ros3_attempt<RADM2_LROWS, RADM2_METHODS != 0>(s, o, kk, y, h2o, m, lu, f, k1, k2, k3);
+      if (s.status == ST_RUNNING)
+        ros3_attempt<RADM2_LROWS, RADM2_METHODS != 0>(s, o, fc, kk, y, h2o, m, lu, f, k1, k2, k3);
     }
   }
   if (valid) cell_store(d, p, y, s, s.hfirst, a.stats);

@@ -28,7 +28,7 @@ struct DomainDesc {
   double* hexit;
   double* hfirst;        // proposal after the first accepted step (learning target)
   double rc_default;
-  double pad;
+  const double* jrate_end;  // [21][ld] J at t = dt (non-autonomous) or null
   Ros3Opts opts;
 };
 
@@ -54,9 +54,9 @@ constexpr int WS_K2 = WS_K1 + NVAR;
 constexpr int WS_K3 = WS_K2 + NVAR;
 // doubles per lane: the Ros3 kernels hold K1..K3; the kernels of KPP's
 // whole Rosenbrock family six stage vectors (Rodas4), stage s at
-// WS_K1 + s*NVAR
+// WS_K1 + s*NVAR, and dF/dT at WS_K1 + 6*NVAR
 constexpr int WS_DOUBLES_ROS3 = WS_K3 + NVAR;
-constexpr int WS_DOUBLES_FAMILY = WS_K1 + 6 * NVAR;
+constexpr int WS_DOUBLES_FAMILY = WS_K1 + 7 * NVAR;
 
 // Placement of the lane-private working arrays
 // (preprocessor tests in the kernels use the RADM2_PLACE_* macros of radm2.h)

@@ -119,8 +119,14 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_persistent(LaunchAr
     }
     if (s.status == ST_RUNNING) {
       const Ros3Opts& o = a.doms[di].opts;
+#if RADM2_METHODS
+      const Forcing fc = cell_forcing(a.doms[di], cell - a.doms[di].first);
+#else
+      const Forcing fc{};
+#endif
       cell_attempt_top(s, o);
-      if (s.status == ST_RUNNING) ros3_attempt<RADM2_LROWS, RADM2_METHODS != 0>(s, o, kk, y, h2o, m, lu, f, k1, k2, k3);
+      if (s.status == ST_RUNNING)
+        ros3_attempt<RADM2_LROWS, RADM2_METHODS != 0>(s, o, fc, kk, y, h2o, m, lu, f, k1, k2, k3);
     }
   }
   if (cell >= 0) {

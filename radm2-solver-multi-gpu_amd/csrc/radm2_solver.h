@@ -48,21 +48,27 @@ constexpr double DELTA_MIN = 1.0e-6;
 // 4 Rodas3, 5 Rodas4 -- coefficients as published (Sandu, Verwer, Blom,
 // Spee, Carmichael and Potra 1997, Atmos. Environ. 31; Hairer and Wanner
 // 1996, RODAS).  A and C hold the strictly lower triangles packed by rows,
-// entry (i, j), j < i, at i*(i-1)/2 + j.  The reference initialises only
-// Ros3 (mgpu.cu:1008-1062, 3802-3811).
+// entry (i, j), j < i, at i*(i-1)/2 + j; alpha: stage times, gamma: row
+// sums of the gamma matrix (the non-autonomous terms).  The reference
+// initialises only Ros3 (mgpu.cu:1008-1062, 3802-3811).
 struct RosMethod {
   int S;
   double A[15], C[15], M[6], E[6];
   int newf[6];
   double gamma1, elo;
+  double alpha[6], gamma[6];
 };
 constexpr double ROS2_G = 1.0 + 1.0 / 1.4142135623730951;  // 1 + 1/sqrt(2)
 __device__ constexpr RosMethod kRosMethods[6] = {
     {},
     {2, {1.0 / ROS2_G}, {-2.0 / ROS2_G}, {3.0 / (2.0 * ROS2_G), 1.0 / (2.0 * ROS2_G)},
-     {1.0 / (2.0 * ROS2_G), 1.0 / (2.0 * ROS2_G)}, {1, 1}, ROS2_G, 2.0},
+     {1.0 / (2.0 * ROS2_G), 1.0 / (2.0 * ROS2_G)}, {1, 1}, ROS2_G, 2.0, {0.0, 1.0},
+     {ROS2_G, -ROS2_G}},
     {3, {1.0, 1.0, 0.0}, {ROS_C21, ROS_C31, ROS_C32}, {ROS_M1, ROS_M2, ROS_M3},
-     {ROS_E1, ROS_E2, ROS_E3}, {1, 1, 0}, ROS_GAMMA1, ROS_ELO},
+     {ROS_E1, ROS_E2, ROS_E3}, {1, 1, 0}, ROS_GAMMA1, ROS_ELO,
+     {0.0, 0.43586652150845899941601945119356, 0.43586652150845899941601945119356},
+     {0.43586652150845899941601945119356, 0.24291996454816804366592249683314,
+      2.1851380027664058511513169485832}},
     {4,
      {2.0, 1.867943637803922, 0.2344449711399156, 1.867943637803922, 0.2344449711399156, 0.0},
      {-7.137615036412310, 2.580708087951457, 0.6515950076447975, -2.137148994382534,
@@ -71,9 +77,12 @@ __device__ constexpr RosMethod kRosMethods[6] = {
      {-0.2815431932141155, -0.07276199124938920, -0.1082196201495311, -1.093502252409163},
      {1, 1, 1, 0},
      0.5728200000000000,
-     4.0},
+     4.0,
+     {0.0, 1.145640000000000, 0.6552168638155900, 0.6552168638155900},
+     {0.5728200000000000, -1.769193891319233, 0.7592633437920482, -0.1049021087100450}},
     {4, {0.0, 2.0, 0.0, 2.0, 0.0, 1.0}, {4.0, 1.0, -1.0, 1.0, -1.0, -(8.0 / 3.0)},
-     {2.0, 0.0, 1.0, 1.0}, {0.0, 0.0, 0.0, 1.0}, {1, 0, 1, 1}, 0.5, 3.0},
+     {2.0, 0.0, 1.0, 1.0}, {0.0, 0.0, 0.0, 1.0}, {1, 0, 1, 1}, 0.5, 3.0, {0.0, 0.0, 1.0, 1.0},
+     {0.5, 1.5, 0.0, 0.0}},
     {6,
      {1.544000000000000, 0.9466785280815826, 0.2557011698983284, 3.314825187068521,
       2.896124015972201, 0.9986419139977817, 1.221224509226641, 6.019134481288629,
@@ -87,7 +96,9 @@ __device__ constexpr RosMethod kRosMethods[6] = {
      {0.0, 0.0, 0.0, 0.0, 0.0, 1.0},
      {1, 1, 1, 1, 1, 1},
      0.25,
-     4.0},
+     4.0,
+     {0.0, 0.386, 0.210, 0.630, 1.0, 1.0},
+     {0.25, -0.1043, 0.1035, -0.3620000000000023e-01, 0.0, 0.0}},
 };
 
 // Per-cell status codes (KPP IERR)
@@ -102,7 +113,9 @@ struct Ros3Opts {
   double dt, rtol, atol;
   double hmin, hmax, hstart, facmin, facmax, facrej, facsafe, roundoff;
   int32_t max_steps;
-  int32_t method;  // KPP's IPAR[3] number: 1 Ros2, 2 Ros3, 3 Ros4, 4 Rodas3, 5 Rodas4
+  int32_t method;        // KPP's IPAR[3] number: 1 Ros2, 2 Ros3, 3 Ros4, 4 Rodas3, 5 Rodas4
+  int32_t sing_retries;  // step halvings after a singular matrix before -8
+  int32_t pad;
 };
 
 __device__ __forceinline__ double dmax_ref(double a, double b) { return a >= b ? a : b; }
@@ -228,20 +241,55 @@ __device__ __forceinline__ void vmap(OA out, Fn fn, const IA&... in) {
   }
 }
 
+// Photolysis forcing of a non-autonomous integration: J(t) = j0 + (t/dt)
+// (j1 - j0) for this cell (j0, j1 at its column, stride ld); j1 null: the
+// integration is autonomous.
+struct Forcing {
+  const double* j0;
+  const double* j1;
+  int64_t ld;
+  double dt;
+};
+
+// The photolysis rate constants (a bare J each, mgpu.cu:763-783) at time t.
+template <class KA>
+__device__ __forceinline__ void photo_at(KA k, const Forcing& fc, double t) {
+  const double w = t / fc.dt;
+#pragma unroll
+  for (int r = 0; r < gen::NPHOTO; r++) {
+    const double a = fc.j0[gen::PHOTO_J[r] * fc.ld], b = fc.j1[gen::PHOTO_J[r] * fc.ld];
+    WR(k, r, fma(w, b - a, a));
+  }
+}
+
+// Step of the dF/dT difference quotient at time T: to the farther end of
+// [0, dt].  The forcing is linear in t, so the quotient is its exact slope
+// for any step; KPP's sqrt(roundoff) max(DeltaMin, |T|) would cancel to
+// noise near T = 0, where every call's time axis starts.
+__device__ __forceinline__ double forcing_delta(double T, double dt) {
+  return T <= 0.5 * dt ? dt - T : -T;
+}
+
 // Stages of one attempt of KPP's generic Rosenbrock scheme on workspace
-// views (the methods other than Ros3): stage s's vector lives in the
-// workspace region k1.region + s*NVAR.  The reference's commented stage
-// loop (mgpu.cu:3595-3625) in KPP's general form: F = f(y) for stage 0,
-// f(y + sum_j A(s,j) K_j) when NewF(s), else the previous F; K_s =
-// G^-1 (F + sum_j C(s,j)/H K_j).  Returns the sum of squared scaled errors
-// (sum_j E_j K_j) and leaves Ynew = y + sum_j M_j K_j in f.  The loops over
-// species are unrolled (constant element indices keep the hot set in LDS
-// and registers); the loops over stages run at run time.
+// views (the methods other than Ros3, and every method when forced): stage
+// s's vector lives in the workspace region k1.region + s*NVAR, dF/dT in
+// region 6.  The reference's commented stage loop (mgpu.cu:3595-3625) in
+// KPP's general form: F = f(T, y) for stage 0, f(T + alpha_s H, y + sum_j
+// A(s,j) K_j) when NewF(s), else the previous F; K_s = G^-1 (F + sum_j
+// C(s,j)/H K_j + H gamma_s dF/dT), the last term when forced (mgpu.cu:
+// 1566-1593, 3619-3622).  The rate constants k hold time T on entry.
+// Returns the sum of squared scaled errors (sum_j E_j K_j) and leaves
+// Ynew = y + sum_j M_j K_j in f.  The loops over species are unrolled
+// (constant element indices keep the hot set in LDS and registers); the
+// loops over stages run at run time.
 template <class KA, class YA, class GA, class VA>
-__device__ __forceinline__ double generic_stages(const RosMethod& rm, KA k, YA y, double h2o,
-                                                 double m, GA g, VA f, VA k1, double H,
-                                                 double rtol, double atol) {
+__device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forcing& fc, double T,
+                                                 KA k, YA y, double h2o, double m, GA g, VA f,
+                                                 VA k1, double H, double rtol, double atol) {
   const int S = rm.S;
+  const bool forced = fc.j1 != nullptr;
+  VA dfdt = k1;
+  dfdt.region = k1.region + 6 * NVAR;
 #pragma nounroll
   for (int st = 0; st < S; st++) {
     const int base = st * (st - 1) / 2;
@@ -249,6 +297,19 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, KA k, YA y
     ks.region = k1.region + st * NVAR;
     if (st == 0) {
       gen::fun_pf16(k, y, h2o, m, f);
+      if (forced) {
+        // dF/dT = (f(T + delta, y) - f(T, y)) / delta
+        const double delta = forcing_delta(T, fc.dt);
+        photo_at(k, fc, T + delta);
+        gen::fun_pf16(k, y, h2o, m, dfdt);
+        // launder copies only: a view laundered inside a branch that lanes
+        // take differently would merge into a per-lane (VGPR) base after it
+        VA d = dfdt, f0 = f;
+        launder(d);
+        launder(f0);
+        const double rd = 1.0 / delta;
+        vmap<16>(d, [rd](int, const double* a) { return fma(-1.0, a[1], a[0]) * rd; }, d, f0);
+      }
     } else if (rm.newf[st]) {
 #pragma unroll
       for (int i = 0; i < NVAR; i++) {
@@ -260,6 +321,7 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, KA k, YA y
         }
         WR(ks, i, v);
       }
+      if (forced) photo_at(k, fc, T + rm.alpha[st] * H);
       gen::fun_pf16(k, ks, h2o, m, f);
     }
 #pragma unroll
@@ -271,6 +333,13 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, KA k, YA y
         v = fma(RD(kj, i), rm.C[base + j] / H, v);
       }
       WR(ks, i, v);
+    }
+    if (forced && rm.gamma[st] != 0.0) {
+      const double hg = H * rm.gamma[st];
+      VA kv = ks, d = dfdt;
+      launder(kv);
+      launder(d);
+      vmap<16>(kv, [hg](int, const double* a) { return fma(hg, a[1], a[0]); }, kv, d);
     }
     launder(ks);
     gen::solve_pf16(g, ks);
@@ -338,20 +407,24 @@ __device__ __forceinline__ void xpass(Fn fn, const IA&... in) {
 // R: launder interval of the compiler-scheduled fused Jacobian/decomposition
 // (rows) and, as 16 reactions per unit, of the tendency on private arrays;
 // phases are laundered apart whenever the arrays are workspace views.
-// FAMILY: the kernel runs every method of KPP's Rosenbrock family (workspace
-// views with six stage regions); otherwise Ros3 alone, whatever o.method.
+// FAMILY: the kernel runs every method of KPP's Rosenbrock family, the
+// non-autonomous forcing fc and KPP's step halving after a singular matrix
+// (workspace views with seven stage regions); otherwise Ros3 alone,
+// autonomous, -8 at once, whatever o and fc ask.
 template <int R, bool FAMILY, class KA, class YA, class GA, class VA>
-__device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA k, YA y,
-                                             double h2o, double m, GA g, VA f, VA k1, VA k2,
-                                             VA k3) {
+__device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, const Forcing& fc,
+                                             KA k, YA y, double h2o, double m, GA g, VA f, VA k1,
+                                             VA k2, VA k3) {
   static_assert(!FAMILY || is_ws_like<GA>::value, "the family needs workspace stage regions");
   constexpr int RF = R > 0 ? 16 * R : 0;
-  const double H = s.H;
-  const bool ros3 = !FAMILY || o.method == 2;
-  const RosMethod& rm = kRosMethods[ros3 ? 2 : o.method];
+  double H = s.H;
+  const bool forced = FAMILY && fc.j1 != nullptr;
+  const bool ros3 = !FAMILY || (o.method == 2 && !forced);
+  const RosMethod& rm = kRosMethods[FAMILY ? o.method : 2];
   const double elo = ros3 ? ROS_ELO : rm.elo;
-  const double ghinv = 1.0 / (H * (ros3 ? ROS_GAMMA1 : rm.gamma1));
-  const double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
+  const double gam = ros3 ? ROS_GAMMA1 : rm.gamma1;
+  const double ghinv = 1.0 / (H * gam);
+  double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
   const double rtol = o.rtol, atol = o.atol;
   double err = 0.0;
   if constexpr (is_ws_like<GA>::value) {
@@ -360,12 +433,31 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, KA
     // in place), and F2 once for the third stage.  The decomposition does
     // not depend on F0, so it comes first.
     // G = -J(y) + ghinv*I formed row by row inside its own LU decomposition
-    if (gen::jac_decomp_pf16(k, y, h2o, m, ghinv, g) != 0) {
-      s.status = ST_SINGULAR;
-      return;
+    if constexpr (FAMILY) {
+      // the Jacobian at T; ros_PrepareMatrix (KPP): a singular matrix halves
+      // H and retries, up to sing_retries times (the reference's active
+      // code: none, mgpu.cu:3147-3163), then -8
+      if (forced) photo_at(k, fc, s.T);
+      for (int nsc = 0;; nsc++) {
+        if (gen::jac_decomp_pf16(k, y, h2o, m, 1.0 / (H * gam), g) == 0) break;
+        if (nsc >= o.sing_retries) {
+          s.H = H;
+          s.status = ST_SINGULAR;
+          return;
+        }
+        H = H * 0.5;
+      }
+      hc21 = ROS_C21 / H;
+      hc31 = ROS_C31 / H;
+      hc32 = ROS_C32 / H;
+    } else {
+      if (gen::jac_decomp_pf16(k, y, h2o, m, ghinv, g) != 0) {
+        s.status = ST_SINGULAR;
+        return;
+      }
     }
     if (!ros3) {
-      err = generic_stages(rm, k, y, h2o, m, g, f, k1, H, rtol, atol);
+      err = generic_stages(rm, fc, s.T, k, y, h2o, m, g, f, k1, H, rtol, atol);
     } else {
     double x[NVAR];
     // K1 = G^-1 f(y)

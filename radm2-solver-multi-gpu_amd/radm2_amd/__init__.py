@@ -54,12 +54,14 @@ class Radm2Error(RuntimeError):
 
 
 class Opts(ctypes.Structure):
-    """radm2_opts: RPAR[0..6], IPAR[2..3] of the reference's Rosenbrock()."""
+    """radm2_opts: RPAR[0..6], IPAR[2..3] of the reference's Rosenbrock(), and
+    sing_retries (KPP's step halvings after a singular matrix; 0 = -8 at
+    once, the reference's active code)."""
     _fields_ = [("hmin", ctypes.c_double), ("hmax", ctypes.c_double),
                 ("hstart", ctypes.c_double), ("facmin", ctypes.c_double),
                 ("facmax", ctypes.c_double), ("facrej", ctypes.c_double),
                 ("facsafe", ctypes.c_double), ("max_steps", ctypes.c_int32),
-                ("method", ctypes.c_int32)]
+                ("method", ctypes.c_int32), ("sing_retries", ctypes.c_int32)]
 
 
 class Domain(ctypes.Structure):
@@ -72,7 +74,7 @@ class Domain(ctypes.Structure):
                 ("status", ctypes.c_void_p), ("hexit", ctypes.c_void_p),
                 ("hfirst", ctypes.c_void_p), ("dt", ctypes.c_double),
                 ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
-                ("opts", ctypes.POINTER(Opts))]
+                ("opts", ctypes.POINTER(Opts)), ("jrate_end", ctypes.c_void_p)]
 
 
 class WrfGrid(ctypes.Structure):
@@ -167,11 +169,13 @@ class CellBatch:
     """Device-resident SoA arrays of one domain (cell index fastest).
 
     conc [59][ld], temp/cair/h2o [ld], jrate [21][ld], optional rc_n2o5 and
-    h0 [ld]; outputs nstep/nacc/nrej/status (int32) and hexit/hfirst (f64).
+    h0 [ld] and jrate_end [21][ld] (J at the end of the interval: a
+    non-autonomous integration); outputs nstep/nacc/nrej/status (int32) and
+    hexit/hfirst (f64).
     """
 
     def __init__(self, conc, temp, cair, h2o, jrate, rc_n2o5=None, h0=None,
-                 outputs=True, hfirst=False):
+                 outputs=True, hfirst=False, jrate_end=None):
         import torch
         self.conc = _need(conc, "f64")
         self.ld = conc.shape[1]
@@ -183,6 +187,8 @@ class CellBatch:
         assert conc.shape[0] == NVAR and jrate.shape == (NJ, self.ld)
         self.rc_n2o5 = None if rc_n2o5 is None else _need(rc_n2o5, "f64")
         self.h0 = None if h0 is None else _need(h0, "f64")
+        self.jrate_end = None if jrate_end is None else _need(jrate_end, "f64")
+        assert jrate_end is None or jrate_end.shape == (NJ, self.ld)
         dev = conc.device
         if outputs:
             self.nstep = torch.zeros(self.ld, dtype=torch.int32, device=dev)
@@ -205,6 +211,7 @@ class CellBatch:
             self.nstep, self.nacc, self.nrej, self.status, self.hexit, self.hfirst))
         d.dt, d.rtol, d.atol = float(dt), float(rtol), float(atol)
         d.opts = ctypes.pointer(opts) if opts is not None else None
+        d.jrate_end = _ptr(self.jrate_end)
         return d
 
 

@@ -22,7 +22,7 @@ class OptsIn(ctypes.Structure):
                 ("hstart", ctypes.c_double), ("facmin", ctypes.c_double),
                 ("facmax", ctypes.c_double), ("facrej", ctypes.c_double),
                 ("facsafe", ctypes.c_double), ("max_steps", ctypes.c_int32),
-                ("method", ctypes.c_int32)]
+                ("method", ctypes.c_int32), ("sing_retries", ctypes.c_int32)]
 
 
 class OptsOut(ctypes.Structure):
@@ -30,7 +30,8 @@ class OptsOut(ctypes.Structure):
                 ("hstart", ctypes.c_double), ("facmin", ctypes.c_double),
                 ("facmax", ctypes.c_double), ("facrej", ctypes.c_double),
                 ("facsafe", ctypes.c_double), ("roundoff", ctypes.c_double),
-                ("max_steps", ctypes.c_int32), ("method", ctypes.c_int32)]
+                ("max_steps", ctypes.c_int32), ("method", ctypes.c_int32),
+                ("sing_retries", ctypes.c_int32)]
 
 
 _lib = None
@@ -61,9 +62,14 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, ctypes.POINTER(OptsOut),
                                        ctypes.c_double, _I, _I, _I, _D]
         L.oracle_ros3_cell.restype = ctypes.c_int
-        L.oracle_ros_cell.argtypes = [ctypes.c_int] + L.oracle_ros3_cell.argtypes
+        L.oracle_ros_cell.argtypes = ([ctypes.c_int, _D, _D, _D, _D]
+                                      + L.oracle_ros3_cell.argtypes[2:])
         L.oracle_ros_cell.restype = ctypes.c_int
-        L.oracle_integrate.argtypes = [ctypes.c_int64, ctypes.c_int64, _D, _D, _D, _D, _D, _D,
+        L.oracle_method_coeffs.argtypes = [ctypes.c_int] + [_D] * 7
+        L.oracle_method_coeffs.restype = None
+        L.oracle_photo_at.argtypes = [_D, _D, _D, ctypes.c_double, ctypes.c_double]
+        L.oracle_photo_at.restype = None
+        L.oracle_integrate.argtypes = [ctypes.c_int64, ctypes.c_int64, _D, _D, _D, _D, _D, _D, _D,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.POINTER(OptsIn), _D, _I, _I, _I, _I, _D,
                                        ctypes.c_int]
@@ -125,8 +131,19 @@ def resolve(dt, rtol, atol, opts=None):
     return rc, out
 
 
-def integrate(cells, dt, rtol, atol, opts=None, h0=None, nthreads=0, n=None, use_rc=True):
-    """Integrate make_cells()-style host arrays; returns (conc, counters dict, code)."""
+def method_coeffs(method):
+    """The oracle's tables of KPP method `method` as numpy arrays."""
+    arr = [np.zeros(15), np.zeros(15), np.zeros(6), np.zeros(6), np.zeros(6), np.zeros(6),
+           np.zeros(2)]
+    lib().oracle_method_coeffs(method, *[_p(a) for a in arr])
+    return dict(zip(("A", "C", "M", "E", "alpha", "gamma", "g_elo"), arr),
+                S=lib().oracle_method_stages(method))
+
+
+def integrate(cells, dt, rtol, atol, opts=None, h0=None, nthreads=0, n=None, use_rc=True,
+              jrate_end=None):
+    """Integrate make_cells()-style host arrays; returns (conc, counters dict, code).
+    jrate_end: J rates at the end of the interval (non-autonomous), or None."""
     conc = np.array(cells["conc"], dtype=np.float64, order="C")
     ld = conc.shape[1]
     n = ld if n is None else n
@@ -137,10 +154,11 @@ def integrate(cells, dt, rtol, atol, opts=None, h0=None, nthreads=0, n=None, use
     hx = np.zeros(ld)
     rc_arr = np.ascontiguousarray(cells["rc_n2o5"]) if use_rc else None
     h0a = None if h0 is None else np.ascontiguousarray(h0, dtype=np.float64)
+    j1 = None if jrate_end is None else np.ascontiguousarray(jrate_end, dtype=np.float64)
     code = lib().oracle_integrate(
         n, ld, _p(conc), _p(np.ascontiguousarray(cells["temp"])),
         _p(np.ascontiguousarray(cells["cair"])), _p(np.ascontiguousarray(cells["h2o"])),
-        _p(np.ascontiguousarray(cells["jrate"])), _p(rc_arr), dt, rtol, atol,
+        _p(np.ascontiguousarray(cells["jrate"])), _p(j1), _p(rc_arr), dt, rtol, atol,
         ctypes.byref(opts) if opts else None, _p(h0a), _p(ns), _p(na), _p(nr), _p(st), _p(hx),
         nthreads)
     return conc, dict(nstep=ns, nacc=na, nrej=nr, status=st, hexit=hx), code

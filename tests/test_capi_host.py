@@ -54,6 +54,7 @@ CASES = {
     "facsafe": [0.0, 0.8, -0.5],
     "max_steps": [0, 50, -1],
     "method": [0, 2, 1, 3, 6, -1],
+    "sing_retries": [0, 5, -1],
 }
 
 
@@ -71,7 +72,10 @@ def _expected(o, rtol, atol):
     if atol <= 0 or rtol <= 10 * 2.0 ** -52 or rtol >= 1:
         return -5
     # KPP's five methods (the reference initialises only Ros3; the others
-    # are implemented here)
+    # are implemented here); the singular-matrix retry count is not a KPP
+    # option: a negative one is an argument error
+    if o["sing_retries"] < 0:
+        return -101
     return 0
 
 

@@ -90,3 +90,139 @@ def test_other_kpp_methods_match_oracle(solver, method):
     finally:
         solver.set_kernel(-1, 0)
         solver.set_schedule(radm2_amd.SCHED_PERSISTENT, 0, 0)
+
+
+def test_mixed_method_domains_on_the_family_kernel(solver):
+    # one batched call with a Ros3 domain and a Rodas3 domain runs on the
+    # family kernel; its Ros3 domain is bit-identical to a Ros3-only call on
+    # the Ros3 kernel, its Rodas3 domain to a Rodas3-only call
+    import radm2_amd
+    from radm2_amd import synth
+    ca = synth.make_cells(300, 41, 100.0, 0.05)
+    cb = synth.make_cells(200, 42, 100.0, 0.05)
+
+    def batch(c):
+        d = synth.to_device(c)
+        return radm2_amd.CellBatch(d["conc"], d["temp"], d["cair"], d["h2o"], d["jrate"],
+                                   rc_n2o5=d["rc_n2o5"])
+
+    def out(b):
+        return [t.cpu().numpy() for t in (b.conc, b.nstep, b.nacc, b.nrej, b.status, b.hexit)]
+
+    a1, b1 = batch(ca), batch(cb)
+    o4 = radm2_amd.Opts(method=4)
+    solver.integrate_batched([a1.domain(600.0, 1e-3, 1.0), b1.domain(300.0, 1e-3, 1.0, opts=o4)])
+    a2, b2 = batch(ca), batch(cb)
+    solver.integrate(a2, 600.0, 1e-3, 1.0)
+    solver.integrate(b2, 300.0, 1e-3, 1.0, opts=o4)
+    solver.sync()
+    for x, y in zip(out(a1) + out(b1), out(a2) + out(b2)):
+        np.testing.assert_array_equal(x, y)
+    assert (b1.status.cpu().numpy() == 1).all()
+    co, oo, code = O.integrate(cb, 300.0, 1e-3, 1.0, opts=O.OptsIn(method=4), nthreads=0)
+    same, ok, rel = _compare(b1.conc.cpu().numpy(), dict(nstep=out(b1)[1], nacc=out(b1)[2],
+                                                         nrej=out(b1)[3]), co, oo, "mixed")
+    assert ok.mean() >= 0.99 and same.mean() >= 0.99
+
+
+def _forced_cells(n, seed):
+    from radm2_amd import synth
+    cells = synth.make_cells(n, seed, 100.0, 0.05)
+    rng = np.random.default_rng(seed)
+    # J at the end of the interval: each cell's rates scaled by U(0, 2)
+    # (sunset to brightening), the same factor for all 21 rates
+    jend = np.ascontiguousarray(cells["jrate"] * rng.uniform(0.0, 2.0, n)[None, :])
+    return cells, jend
+
+
+@pytest.mark.parametrize("method", [2, 4])
+def test_nonautonomous_forcing_matches_oracle(solver, method):
+    # time-varying photolysis J(t) = J0 + (t/dt)(J1 - J0) with dF/dT in every
+    # stage, on the family kernel, persistent and grid schedules
+    import torch
+    import radm2_amd
+    from radm2_amd import synth
+    cells, jend = _forced_cells(256, 50 + method)
+    co, oo, code = O.integrate(cells, 600.0, 1e-3, 1.0, opts=O.OptsIn(method=method),
+                               jrate_end=jend, nthreads=0)
+    assert code == 0 and (oo["status"] == 1).all()
+    # the forcing matters: the autonomous run ends elsewhere
+    ca, _, _ = O.integrate(cells, 600.0, 1e-3, 1.0, opts=O.OptsIn(method=method), nthreads=0)
+    assert np.abs(ca - co).max() > 1e-3 * np.abs(co).max()
+    for sched in (radm2_amd.SCHED_PERSISTENT, radm2_amd.SCHED_GRID):
+        dev = synth.to_device(cells)
+        b = radm2_amd.CellBatch(dev["conc"], dev["temp"], dev["cair"], dev["h2o"], dev["jrate"],
+                                rc_n2o5=dev["rc_n2o5"],
+                                jrate_end=torch.from_numpy(jend).cuda())
+        solver.set_schedule(sched)
+        solver.integrate(b, 600.0, 1e-3, 1.0, opts=radm2_amd.Opts(method=method))
+        solver.sync()
+        og = dict(nstep=b.nstep.cpu().numpy(), nacc=b.nacc.cpu().numpy(),
+                  nrej=b.nrej.cpu().numpy())
+        assert (b.status.cpu().numpy() == 1).all()
+        same, ok, rel = _compare(b.conc.cpu().numpy(), og, co, oo,
+                                 "forced method %d schedule %d" % (method, sched))
+        assert ok.mean() >= 0.99 and same.mean() >= 0.99
+    solver.set_schedule(radm2_amd.SCHED_PERSISTENT)
+
+
+@pytest.mark.parametrize("method", [2, 4])
+def test_singular_matrix_retries_match_oracle(solver, method):
+    # exact zero pivots at H = 500 s (all three cells) and 250 s (cell 0):
+    # -8 at once without retries, cell 0 alone with one, none with KPP's five
+    import torch
+    import radm2_amd
+    from radm2_amd import synth
+    from test_integrator_oracle import _singular_cells
+    cells = _singular_cells(1, method)
+    h0 = np.full(3, 500.0)
+    for retries, want in ((0, [-8, -8, -8]), (1, [-8, 1, 1]), (5, [1, 1, 1])):
+        co, oo, code = O.integrate(cells, 600.0, 1e-3, 1.0, h0=h0, nthreads=0,
+                                   opts=O.OptsIn(method=method, sing_retries=retries))
+        assert code == 0 and list(oo["status"]) == want
+        dev = synth.to_device(cells)
+        b = radm2_amd.CellBatch(dev["conc"], dev["temp"], dev["cair"], dev["h2o"], dev["jrate"],
+                                rc_n2o5=dev["rc_n2o5"], h0=torch.from_numpy(h0).cuda())
+        solver.integrate(b, 600.0, 1e-3, 1.0,
+                         opts=radm2_amd.Opts(method=method, sing_retries=retries))
+        solver.sync()
+        st = b.status.cpu().numpy()
+        assert list(st) == want, (retries, st)
+        for key in ("nstep", "nacc", "nrej"):
+            np.testing.assert_array_equal(getattr(b, key).cpu().numpy(), oo[key], err_msg=key)
+        okc = st == 1
+        hx = b.hexit.cpu().numpy()
+        np.testing.assert_array_equal(hx[~okc], oo["hexit"][~okc])    # 500 or 250 s exactly
+        np.testing.assert_allclose(hx[okc], oo["hexit"][okc], rtol=1e-6)
+        if okc.any():
+            big = np.abs(co[:, okc]) > 1e3
+            cg = b.conc.cpu().numpy()[:, okc]
+            assert (np.abs(cg - co[:, okc])[big] <= 1e-6 * np.abs(co[:, okc])[big]).all()
+
+
+def test_kpp_frontend_nonautonomous(solver):
+    # IPAR[0] = 0 over the domain's jrate_end: the same integration as the
+    # batched entry point with the forcing, one more Fun per step for dF/dT
+    import torch
+    import radm2_amd
+    from radm2_amd import synth
+    cells, jend = _forced_cells(200, 60)
+
+    def batch():
+        dev = synth.to_device(cells)
+        return radm2_amd.CellBatch(dev["conc"], dev["temp"], dev["cair"], dev["h2o"],
+                                   dev["jrate"], rc_n2o5=dev["rc_n2o5"],
+                                   jrate_end=torch.from_numpy(jend).cuda())
+
+    a, b = batch(), batch()
+    RPAR = [0.0] * 20
+    IPAR = [0] * 20
+    IPAR[3] = 2
+    RPAR[2] = 6.0
+    assert solver.rosenbrock(a, 0.0, 600.0, [1.0] * 59, [1e-3] * 59, RPAR, IPAR) == 1
+    solver.integrate(b, 600.0, 1e-3, 1.0, opts=radm2_amd.Opts(hstart=6.0))
+    solver.sync()
+    np.testing.assert_array_equal(a.conc.cpu().numpy(), b.conc.cpu().numpy())
+    nacc, nstp = int(b.nacc.sum()), int(b.nstep.sum())
+    assert IPAR[12] == nstp and IPAR[13] == nacc
+    assert IPAR[10] == 2 * nacc + nstp

@@ -178,16 +178,21 @@ def test_per_cell_h0_is_a_user_hstart(h):
         assert d[2:] == b[2:]
 
 
-def _method_cell(method, y, k, dt, rtol, atol, opts=None, h2o=1e17, m=2.5e19):
+def _method_cell(method, y, k, dt, rtol, atol, opts=None, h2o=1e17, m=2.5e19, j0=None, j1=None,
+                 h0=0.0, hexit=False):
     rc, oo = O.resolve(dt, rtol, atol, opts)
     assert rc == 0
     y = np.array(y, dtype=np.float64)
     ns, na, nr = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
     hx = ctypes.c_double()
     kk = np.ascontiguousarray(k, dtype=np.float64)
-    st = O.lib().oracle_ros_cell(method, O._p(y), O._p(kk), h2o, m, dt, rtol, atol,
-                                 ctypes.byref(oo), 0.0, ctypes.byref(ns), ctypes.byref(na),
+    j0 = None if j0 is None else np.ascontiguousarray(j0, dtype=np.float64)
+    j1 = None if j1 is None else np.ascontiguousarray(j1, dtype=np.float64)
+    st = O.lib().oracle_ros_cell(method, O._p(y), O._p(kk), O._p(j0), O._p(j1), h2o, m, dt, rtol,
+                                 atol, ctypes.byref(oo), h0, ctypes.byref(ns), ctypes.byref(na),
                                  ctypes.byref(nr), ctypes.byref(hx))
+    if hexit:
+        return st, y, ns.value, na.value, nr.value, hx.value
     return st, y, ns.value, na.value, nr.value
 
 
@@ -268,3 +273,148 @@ def test_kpp_methods_stable_on_stiff_decay_and_real_cells(method):
         assert st == 1
         big = np.abs(ref) > 1e3
         assert np.max(np.abs(y[big] - ref[big]) / np.abs(ref[big])) < 1e-3
+
+
+def _tri(v, S):
+    m = np.zeros((S, S))
+    q = 0
+    for i in range(1, S):
+        for j in range(i):
+            m[i, j] = v[q]
+            q += 1
+    return m
+
+
+@pytest.mark.parametrize("method", [1, 2, 3, 4, 5])
+def test_stage_times_and_gamma_sums_follow_from_the_method(method):
+    # KPP's transformed coefficients: C = diag(1/gamma) - Gamma^-1 and
+    # A = alpha Gamma^-1, so gamma_i (row sums of Gamma) and alpha_i (row sums
+    # of A Gamma) follow from A, C and gamma, which the order tests pin
+    c = O.method_coeffs(method)
+    S, g = c["S"], c["g_elo"][0]
+    G = np.linalg.inv(np.eye(S) / g - _tri(c["C"], S))
+    np.testing.assert_allclose(c["gamma"][:S], G.sum(axis=1), rtol=0, atol=5e-15)
+    np.testing.assert_allclose(c["alpha"][:S], (_tri(c["A"], S) @ G).sum(axis=1), rtol=0,
+                               atol=5e-15)
+    if method == 2:
+        # the reference's Ros3 tables (mgpu.cu:1054-1061)
+        assert list(c["alpha"][:3]) == [0.0, 0.43586652150845899941601945119356,
+                                        0.43586652150845899941601945119356]
+        assert list(c["gamma"][:3]) == [0.43586652150845899941601945119356,
+                                        0.24291996454816804366592249683314,
+                                        2.1851380027664058511513169485832]
+
+
+def _forced_no2(method, h=None, j0v=8e-3, j1v=0.0, rtol=1e-3, autonomous=False):
+    # NO2 photolysis with J(t) = j0 + (j1 - j0) t/dt: NO2(dt) = NO2(0) exp(-(j0 + j1) dt/2)
+    dt = 600.0
+    y0 = np.zeros(59)
+    y0[NO2], y0[NO] = 1e11, 3e10
+    j0, j1 = np.zeros(21), np.zeros(21)
+    j0[2], j1[2] = j0v, j1v
+    k = np.zeros(156)
+    k[0] = j0v
+    opts = _opts(hmin=h, hstart=h, facmin=1.0, facmax=1.0) if h else None
+    st, y, ns, na, nr = _method_cell(method, y0, k, dt, rtol, 1.0, opts, j0=j0,
+                                     j1=None if autonomous else j1)
+    return st, y, ns, na, nr, 1e11 * np.exp(-(j0v + j1v) * dt / 2.0)
+
+
+@pytest.mark.parametrize("method,order", [(1, 2), (2, 3), (3, 4), (4, 3), (5, 4)])
+def test_nonautonomous_observed_order(method, order):
+    # fixed steps against the closed form of photolysis under a linearly
+    # falling J: the stage times alpha and the gamma sums enter the order
+    # conditions of a non-autonomous problem, so a wrong entry breaks the
+    # method's order
+    errs = []
+    for h in (12.5, 6.25, 3.125):
+        st, y, ns, na, nr, exact = _forced_no2(method, h)
+        assert st == 1 and nr == 0 and na == int(round(600.0 / h))
+        errs.append(abs(y[NO2] - exact) / exact)
+    orders = [np.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert all(order - 0.5 < p < order + 0.5 for p in orders), (errs, orders)
+
+
+@pytest.mark.parametrize("method", [1, 2, 3, 4, 5])
+def test_nonautonomous_adaptive_and_constant_forcing(method):
+    # adaptive integration under the falling J meets the tolerance; a
+    # constant J (j1 = j0) is the autonomous problem bit for bit (dF/dT = 0)
+    st, y, ns, na, nr, exact = _forced_no2(method, rtol=1e-6)
+    assert st == 1 and abs(y[NO2] - exact) / exact < 1e-4
+    a = _forced_no2(method, j1v=8e-3)
+    b = _forced_no2(method, j1v=8e-3, autonomous=True)
+    assert a[0] == b[0] == 1 and a[2:5] == b[2:5]
+    np.testing.assert_array_equal(a[1], b[1])
+    # the autonomous run ignores the J rates of the forcing: the same as the
+    # Ros3 path with k[0] = j
+    if method == 2:
+        k = np.zeros(156)
+        k[0] = 8e-3
+        y0 = np.zeros(59)
+        y0[NO2], y0[NO] = 1e11, 3e10
+        c = _cell(y0, k, 600.0, 1e-3, 1.0)
+        np.testing.assert_array_equal(c[1], b[1])
+
+
+def _singular_cells(n_h2o2, method=2):
+    # three cells whose matrix G = -J + I/(H gamma) has an exactly zero pivot
+    # at H = 500 s (HONO: its only loss is photolysis J[5], and row 12 has no
+    # L entries) and, for the first n_h2o2 cells, at H = 250 s too (H2O2:
+    # J[8] and OH + H2O2, with OH = 0; row 13 has no L entries either)
+    from radm2_amd import synth
+    c = synth.make_cells(4, 3, 60.0, 0.0)
+    c = {key: np.ascontiguousarray(v[..., 1:]) for key, v in c.items()}
+    g = O.method_coeffs(method)["g_elo"][0]
+    for p in range(3):
+        c["jrate"][5, p] = -1.0 / (500.0 * g)
+        c["conc"][MECH["species"].index("HONO"), p] = 10.0
+        if p < n_h2o2:
+            c["jrate"][8, p] = -1.0 / (250.0 * g)
+            c["conc"][MECH["species"].index("H2O2"), p] = 10.0
+            c["conc"][MECH["species"].index("OH"), p] = 0.0
+    return c
+
+
+@pytest.mark.parametrize("method", [2, 4])
+def test_singular_pivots_are_exact_zeros(method):
+    c = _singular_cells(3, method)
+    g = O.method_coeffs(method)["g_elo"][0]
+    p = 0
+    k = O.rconst(c["temp"][p], c["cair"][p], c["jrate"][:, p], c["rc_n2o5"][p])
+    for H, row in ((500.0, 12), (250.0, 13)):
+        G = O.jac_prep(c["conc"][:, p], c["h2o"][p], c["cair"][p], k, 1.0 / (H * g))
+        assert G[MECH["lu"]["diag"][row]] == 0.0
+        rc, _ = O.decomp(G)
+        assert rc == row + 1
+
+
+@pytest.mark.parametrize("method", [2, 4])
+def test_singular_matrix_step_halving(method):
+    h0 = np.full(3, 500.0)
+    c = _singular_cells(1, method)
+    # the reference's active code: -8 at the first decomposition, at H0
+    co, oo, code = O.integrate(c, 600.0, 1e-3, 1.0, opts=_opts(method=method), h0=h0)
+    assert code == 0 and (oo["status"] == -8).all() and (oo["nstep"] == 0).all()
+    assert (oo["hexit"] == 500.0).all()
+    # one halving allowed: cells 1, 2 recover at 250 s; cell 0 meets the
+    # second zero pivot there and ends with -8 at H = 250
+    co, oo, code = O.integrate(c, 600.0, 1e-3, 1.0, opts=_opts(method=method, sing_retries=1),
+                               h0=h0)
+    assert list(oo["status"]) == [-8, 1, 1] and oo["hexit"][0] == 250.0
+    # KPP's five halvings: every cell recovers
+    co, oo, code = O.integrate(c, 600.0, 1e-3, 1.0, opts=_opts(method=method, sing_retries=5),
+                               h0=h0)
+    assert (oo["status"] == 1).all()
+    # without a singular matrix the retries change nothing
+    from radm2_amd import synth
+    d = synth.make_cells(8, 4, 100.0, 0.05)
+    a = O.integrate(d, 600.0, 1e-3, 1.0, opts=_opts(method=method))
+    b = O.integrate(d, 600.0, 1e-3, 1.0, opts=_opts(method=method, sing_retries=5))
+    np.testing.assert_array_equal(a[0], b[0])
+    for key in a[1]:
+        np.testing.assert_array_equal(a[1][key], b[1][key])
+
+
+def test_negative_singular_retries_rejected():
+    rc, _ = O.resolve(600.0, 1e-3, 1.0, _opts(sing_retries=-1))
+    assert rc == -101
