@@ -217,11 +217,42 @@ def main():
                          "h0_floor_s": 1e-6 if name.endswith("1e-6") else 1e-12}
     if not args.no_extra:
         out["end_to_end"] = end_to_end(solver, cells, args, torch)
+        out["family"] = family(solver, batch, conc0, args, torch)
     if na > 0 and not args.no_extra:
         out["accuracy"] = accuracy(solver, dev, conc0, ends, na, args, torch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cells, args)
     print(json.dumps(out), flush=True)
+
+
+def family(solver, batch, conc0, args, torch):
+    """KPP's other Rosenbrock methods and the non-autonomous forcing on the
+    family kernel: one warm-up and one timed step each on the same cells
+    (the forcing: every cell's J falls linearly to half over the interval)."""
+    import radm2_amd
+    n = batch.ncells
+    out = {}
+    cases = [("ros2", 1, False), ("ros4", 3, False), ("rodas3", 4, False), ("rodas4", 5, False),
+             ("ros3_forced", 2, True)]
+    solver.set_schedule(radm2_amd.SCHED_PERSISTENT, args.block, args.refill)
+    for name, method, forced in cases:
+        batch.jrate_end = (0.5 * batch.jrate).contiguous() if forced else None
+        opts = radm2_amd.Opts(method=method)
+        for rep in range(2):
+            batch.conc.copy_(conc0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            solver.integrate(batch, args.dt, args.rtol, args.atol, opts=opts)
+            solver.sync()
+            wall = time.perf_counter() - t0
+        out[name] = {"cells_per_s": n / wall, "kernel_ms": solver.elapsed_ms(),
+                     "attempts_mean": float(batch.nstep.double().mean()),
+                     "cells_ok": int((batch.status == radm2_amd.CELL_OK).sum())}
+        print("[bench] family %s: %.1f ms, attempts mean %.1f, ok %d" % (
+            name, out[name]["kernel_ms"], out[name]["attempts_mean"], out[name]["cells_ok"]),
+            file=sys.stderr, flush=True)
+    batch.jrate_end = None
+    return out
 
 
 def end_to_end(solver, cells, args, torch):

@@ -560,14 +560,16 @@ int radm2_rosenbrock(radm2_ctx* c, const radm2_domain* state, double Tstart, dou
   if (int rc = resolve_opts(dt, RelTol[0], AbsTol[0], &o, &ro)) return rc;
   const int64_t n = state->ncells;
   if (n <= 0) return RADM2_E_ARG;
-  int32_t *ns = nullptr, *na = nullptr, *nr = nullptr, *st = nullptr;
-  double* hx = nullptr;
+  // one allocation for the per-cell outputs: hexit first (8-byte aligned),
+  // then the four int32 arrays
   HIP_TRY(hipSetDevice(c->device));
-  HIP_TRY(hipMalloc(&ns, n * sizeof(int32_t)));
-  HIP_TRY(hipMalloc(&na, n * sizeof(int32_t)));
-  HIP_TRY(hipMalloc(&nr, n * sizeof(int32_t)));
-  HIP_TRY(hipMalloc(&st, n * sizeof(int32_t)));
-  HIP_TRY(hipMalloc(&hx, n * sizeof(double)));
+  void* buf = nullptr;
+  HIP_TRY(hipMalloc(&buf, (size_t)n * (sizeof(double) + 4 * sizeof(int32_t))));
+  double* hx = (double*)buf;
+  int32_t* ns = (int32_t*)(hx + n);
+  int32_t* na = ns + n;
+  int32_t* nr = na + n;
+  int32_t* st = nr + n;
   radm2_domain d = *state;
   d.nstep = ns;
   d.nacc = na;
@@ -618,11 +620,7 @@ int radm2_rosenbrock(radm2_ctx* c, const radm2_domain* state, double Tstart, dou
   } else if (rc == 0) {
     rc = RADM2_E_HIP;
   }
-  hipFree(ns);
-  hipFree(na);
-  hipFree(nr);
-  hipFree(st);
-  hipFree(hx);
+  hipFree(buf);
   return rc;
 }
 

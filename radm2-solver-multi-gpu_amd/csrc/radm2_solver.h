@@ -273,15 +273,17 @@ __device__ __forceinline__ double forcing_delta(double T, double dt) {
 // Stages of one attempt of KPP's generic Rosenbrock scheme on workspace
 // views (the methods other than Ros3, and every method when forced): stage
 // s's vector lives in the workspace region k1.region + s*NVAR, dF/dT in
-// region 6.  The reference's commented stage loop (mgpu.cu:3595-3625) in
-// KPP's general form: F = f(T, y) for stage 0, f(T + alpha_s H, y + sum_j
-// A(s,j) K_j) when NewF(s), else the previous F; K_s = G^-1 (F + sum_j
-// C(s,j)/H K_j + H gamma_s dF/dT), the last term when forced (mgpu.cu:
-// 1566-1593, 3619-3622).  The rate constants k hold time T on entry.
-// Returns the sum of squared scaled errors (sum_j E_j K_j) and leaves
-// Ynew = y + sum_j M_j K_j in f.  The loops over species are unrolled
-// (constant element indices keep the hot set in LDS and registers); the
-// loops over stages run at run time.
+// region 6, the latest new F in f.  The reference's commented stage loop
+// (mgpu.cu:3595-3625) in KPP's general form: F = f(T, y) for stage 0,
+// f(T + alpha_s H, y + sum_j A(s,j) K_j) when NewF(s), else the previous F;
+// K_s = G^-1 (F + sum_j C(s,j)/H K_j + H gamma_s dF/dT), the last term when
+// forced (mgpu.cu:1566-1593, 3619-3622).  The rate constants k hold time T
+// on entry.  As in the fused Ros3 stages, each right-hand side is formed in
+// registers (the tendency writes it there) and solved there; a stage's K
+// is stored once.  The loops over stages and over earlier stages run at run
+// time, the loops over species are unrolled (constant element indices keep
+// x and the hot set in registers and LDS).  Returns the sum of squared
+// scaled errors (sum_j E_j K_j) and leaves Ynew = y + sum_j M_j K_j in f.
 template <class KA, class YA, class GA, class VA>
 __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forcing& fc, double T,
                                                  KA k, YA y, double h2o, double m, GA g, VA f,
@@ -295,55 +297,70 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
     const int base = st * (st - 1) / 2;
     VA ks = k1;
     ks.region = k1.region + st * NVAR;
-    if (st == 0) {
-      gen::fun_pf16(k, y, h2o, m, f);
-      if (forced) {
-        // dF/dT = (f(T + delta, y) - f(T, y)) / delta
-        const double delta = forcing_delta(T, fc.dt);
-        photo_at(k, fc, T + delta);
-        gen::fun_pf16(k, y, h2o, m, dfdt);
-        // launder copies only: a view laundered inside a branch that lanes
-        // take differently would merge into a per-lane (VGPR) base after it
-        VA d = dfdt, f0 = f;
-        launder(d);
-        launder(f0);
-        const double rd = 1.0 / delta;
-        vmap<16>(d, [rd](int, const double* a) { return fma(-1.0, a[1], a[0]) * rd; }, d, f0);
-      }
-    } else if (rm.newf[st]) {
+    double x[NVAR];
+    if (st == 0 || rm.newf[st]) {
+      if (st == 0) {
+        gen::fun_pf16_reg(k, y, h2o, m, x);
+      } else {
+        // Ynew = y + sum_j A(s,j) K_j, accumulated in x, stored for the tendency
 #pragma unroll
-      for (int i = 0; i < NVAR; i++) {
-        double v = RD(y, i);
+        for (int i = 0; i < NVAR; i++) x[i] = RD(y, i);
         for (int j = 0; j < st; j++) {
           VA kj = k1;
           kj.region = k1.region + j * NVAR;
-          v = fma(rm.A[base + j], RD(kj, i), v);
-        }
-        WR(ks, i, v);
-      }
-      if (forced) photo_at(k, fc, T + rm.alpha[st] * H);
-      gen::fun_pf16(k, ks, h2o, m, f);
-    }
+          const double a = rm.A[base + j];
 #pragma unroll
-    for (int i = 0; i < NVAR; i++) {
-      double v = RD(f, i);
-      for (int j = 0; j < st; j++) {
-        VA kj = k1;
-        kj.region = k1.region + j * NVAR;
-        v = fma(RD(kj, i), rm.C[base + j] / H, v);
+          for (int i = 0; i < NVAR; i++) x[i] = fma(a, RD(kj, i), x[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < NVAR; i++) WR(ks, i, x[i]);
+        launder(ks);
+        if (forced) photo_at(k, fc, T + rm.alpha[st] * H);
+        gen::fun_pf16_reg(k, ks, h2o, m, x);
       }
-      WR(ks, i, v);
+#pragma unroll
+      for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
+      if (st == 0 && forced) {
+        // dF/dT = (f(T + delta, y) - f(T, y)) / delta; x back to F0 after
+        const double delta = forcing_delta(T, fc.dt);
+        const double rd = 1.0 / delta;
+        photo_at(k, fc, T + delta);
+        gen::fun_pf16_reg(k, y, h2o, m, x);
+        VA f0 = f;
+        launder(f0);
+#pragma unroll
+        for (int i = 0; i < NVAR; i++) {
+          const double fi = RD(f0, i);
+          WR(dfdt, i, fma(-1.0, fi, x[i]) * rd);
+          x[i] = fi;
+        }
+      }
+    } else {
+      VA fv = f;
+      launder(fv);
+#pragma unroll
+      for (int i = 0; i < NVAR; i++) x[i] = RD(fv, i);
+    }
+    // + sum_j C(s,j)/H K_j, then H gamma_s dF/dT
+    for (int j = 0; j < st; j++) {
+      VA kj = k1;
+      kj.region = k1.region + j * NVAR;
+      const double c = rm.C[base + j] / H;
+#pragma unroll
+      for (int i = 0; i < NVAR; i++) x[i] = fma(RD(kj, i), c, x[i]);
     }
     if (forced && rm.gamma[st] != 0.0) {
       const double hg = H * rm.gamma[st];
-      VA kv = ks, d = dfdt;
-      launder(kv);
+      VA d = dfdt;
       launder(d);
-      vmap<16>(kv, [hg](int, const double* a) { return fma(hg, a[1], a[0]); }, kv, d);
+#pragma unroll
+      for (int i = 0; i < NVAR; i++) x[i] = fma(hg, RD(d, i), x[i]);
     }
-    launder(ks);
-    gen::solve_pf16(g, ks);
+    gen::solve_pf16_reg(g, x);
+#pragma unroll
+    for (int i = 0; i < NVAR; i++) WR(ks, i, x[i]);
   }
+  launder(k1);
   double err = 0.0;
 #pragma unroll
   for (int i = 0; i < NVAR; i++) {
@@ -420,9 +437,13 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
   double H = s.H;
   const bool forced = FAMILY && fc.j1 != nullptr;
   const bool ros3 = !FAMILY || (o.method == 2 && !forced);
+  // the family reads its scalars from the table for every lane (Ros3's row
+  // holds Ros3's values): a per-lane choice between a constant and a load
+  // would be a branch whose merged value the compiler may spill under the
+  // branch's lane mask only, leaving the other lanes' copy unset
   const RosMethod& rm = kRosMethods[FAMILY ? o.method : 2];
-  const double elo = ros3 ? ROS_ELO : rm.elo;
-  const double gam = ros3 ? ROS_GAMMA1 : rm.gamma1;
+  const double elo = FAMILY ? rm.elo : ROS_ELO;
+  const double gam = FAMILY ? rm.gamma1 : ROS_GAMMA1;
   const double ghinv = 1.0 / (H * gam);
   double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
   const double rtol = o.rtol, atol = o.atol;
