@@ -226,3 +226,40 @@ def test_kpp_frontend_nonautonomous(solver):
     nacc, nstp = int(b.nacc.sum()), int(b.nstep.sum())
     assert IPAR[12] == nstp and IPAR[13] == nacc
     assert IPAR[10] == 2 * nacc + nstp
+
+
+def test_mixed_forcing_methods_and_retries_in_one_call(solver):
+    # one batched call on the family kernel whose waves mix a forced Ros3
+    # domain, an autonomous Rodas4 domain and an autonomous Ros3 domain with
+    # retries enabled: each domain bit-identical to its own single-domain call
+    import torch
+    import radm2_amd
+    from radm2_amd import synth
+    ca, ja = _forced_cells(150, 71)
+    cb = synth.make_cells(130, 72, 100.0, 0.05)
+    cc = synth.make_cells(110, 73, 100.0, 0.05)
+
+    def batch(c, jend=None):
+        d = synth.to_device(c)
+        return radm2_amd.CellBatch(d["conc"], d["temp"], d["cair"], d["h2o"], d["jrate"],
+                                   rc_n2o5=d["rc_n2o5"],
+                                   jrate_end=None if jend is None else torch.from_numpy(jend).cuda())
+
+    oa, ob, oc = radm2_amd.Opts(), radm2_amd.Opts(method=5), radm2_amd.Opts(sing_retries=5)
+    for sched in (radm2_amd.SCHED_PERSISTENT, radm2_amd.SCHED_GRID):
+        solver.set_schedule(sched)
+        a1, b1, c1 = batch(ca, ja), batch(cb), batch(cc)
+        solver.integrate_batched([a1.domain(600.0, 1e-3, 1.0, opts=oa),
+                                  b1.domain(600.0, 1e-3, 1.0, opts=ob),
+                                  c1.domain(600.0, 1e-3, 1.0, opts=oc)])
+        a2, b2, c2 = batch(ca, ja), batch(cb), batch(cc)
+        solver.integrate(a2, 600.0, 1e-3, 1.0, opts=oa)
+        solver.integrate(b2, 600.0, 1e-3, 1.0, opts=ob)
+        solver.integrate(c2, 600.0, 1e-3, 1.0, opts=radm2_amd.Opts())   # Ros3 kernel
+        solver.sync()
+        for x, y in ((a1, a2), (b1, b2), (c1, c2)):
+            for key in ("conc", "nstep", "nacc", "nrej", "status", "hexit"):
+                np.testing.assert_array_equal(getattr(x, key).cpu().numpy(),
+                                              getattr(y, key).cpu().numpy(), err_msg=key)
+            assert (x.status.cpu().numpy() == 1).all()
+    solver.set_schedule(radm2_amd.SCHED_PERSISTENT)
