@@ -198,7 +198,7 @@ def main():
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic_bytes(r),
                      "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE per attempt, calibrated "
-                                       "with a 1 GiB copy (profiles/radm2_persistent_1m.md), "
+                                       "with a 1 GiB copy (profiles/radm2_persistent_lds160_reg128_1m.md), "
                                        "scaled to this launch's attempts",
                      "flop_per_launch": flop, "kernel_ms": kms,
                      "hbm_gbs": (traffic_bytes(r) or 0) / (kms * 1e-3) / 1e9},
