@@ -263,3 +263,34 @@ def test_mixed_forcing_methods_and_retries_in_one_call(solver):
                                               getattr(y, key).cpu().numpy(), err_msg=key)
             assert (x.status.cpu().numpy() == 1).all()
     solver.set_schedule(radm2_amd.SCHED_PERSISTENT)
+
+
+def test_family_plan_replays_match_eager(solver):
+    # a captured plan of a forced Rodas3 domain replays bit-identically to the
+    # eager call, with eager work between the replays
+    import torch
+    import radm2_amd
+    from radm2_amd import synth
+    cells, jend = _forced_cells(200, 80)
+    d = synth.to_device(cells)
+    b = radm2_amd.CellBatch(d["conc"], d["temp"], d["cair"], d["h2o"], d["jrate"],
+                            rc_n2o5=d["rc_n2o5"], jrate_end=torch.from_numpy(jend).cuda())
+    conc0 = b.conc.clone()
+    o = radm2_amd.Opts(method=4)
+    solver.integrate(b, 600.0, 1e-3, 1.0, opts=o)
+    solver.sync()
+    eager = [t.cpu().numpy() for t in (b.conc, b.nstep, b.nacc, b.nrej, b.status, b.hexit)]
+    assert (eager[4] == 1).all()
+    plan = solver.plan([b.domain(600.0, 1e-3, 1.0, opts=o)])
+    for rep in range(3):
+        b.conc.copy_(conc0)
+        b.nstep.zero_()
+        torch.cuda.synchronize()
+        plan.launch()
+        solver.sync()
+        got = [t.cpu().numpy() for t in (b.conc, b.nstep, b.nacc, b.nrej, b.status, b.hexit)]
+        for x, y in zip(got, eager):
+            np.testing.assert_array_equal(x, y, err_msg="replay %d" % rep)
+        torch.cuda.synchronize()
+        _ = (b.conc * 2.0).sum().item()
+    plan.close()
