@@ -7,25 +7,36 @@ lanes that take the branch, skip the block when none do, and restore exec
 at the join.  A value merged at the join (assigned on both sides) and then
 spilled must be written for every lane; a spill store or AGPR copy emitted
 in the join block *before* exec is restored covers only the branch's lanes,
-and the other lanes later reload a stale copy.  Two builds of these kernels
+and the other lanes later reload a stale copy.  (An execz test not preceded
+by a narrowing of exec is a long-branch relaxation, not a lane mask, and is
+not scanned.)  Two builds of these kernels
 showed exactly that (every cell of a kernel variant went wrong).  This check
 flags any scratch store or AGPR write between an execz-branch target label
-and the next exec restore, branch or label.
+and the next write of exec, branch or label.
 
 Usage: check_exec_spills.py file.s [...]; exit status 1 if any is flagged.
 """
 import re
 import sys
 
-BARRIER = re.compile(r"s_or_b64\s+exec, exec|s_andn2_saveexec|s_and_saveexec|s_cbranch|^\.LBB")
+# anything that writes exec, a branch or a label ends the join block
+BARRIER = re.compile(r"s_\w+_saveexec_b64|s_\w+_b64\s+exec,|s_cbranch|^\.LBB")
+
+
+NARROW = re.compile(r"s_(and|andn2|or|xor)_saveexec_b64|s_(mov|and|andn2|xor)_b64\s+exec,")
 
 
 def scan(path):
     lines = open(path).read().split("\n")
     targets = set()
-    for ln in lines:
+    for k, ln in enumerate(lines):
         mo = re.search(r"s_cbranch_execz\s+(\S+)", ln)
-        if mo:
+        if not mo:
+            continue
+        # a branch around a block run by a subset of lanes: exec was narrowed
+        # just before (an execz test without that is a branch relaxation)
+        prev = [x for x in lines[max(0, k - 8):k] if x.strip() and not x.strip().startswith(";")]
+        if any(NARROW.search(x) for x in prev[-4:]):
             targets.add(mo.group(1))
     found = []
     for i, ln in enumerate(lines):

@@ -280,9 +280,12 @@ __device__ __forceinline__ double forcing_delta(double T, double dt) {
 // forced (mgpu.cu:1566-1593, 3619-3622).  The rate constants k hold time T
 // on entry.  As in the fused Ros3 stages, each right-hand side is formed in
 // registers (the tendency writes it there) and solved there; a stage's K
-// is stored once.  The loops over stages and over earlier stages run at run
-// time, the loops over species are unrolled (constant element indices keep
-// x and the hot set in registers and LDS).  Returns the sum of squared
+// is stored once.  The caller hands one method at a time, the same for
+// every lane that runs it, so the coefficients come through the scalar
+// cache.  The loops over stages and over earlier stages run at run time
+// (unrolling them for each stage count was slower: code size and spills);
+// the loops over species are unrolled (constant element indices keep x and
+// the hot set in registers and LDS).  Returns the sum of squared
 // scaled errors (sum_j E_j K_j) and leaves Ynew = y + sum_j M_j K_j in f.
 template <class KA, class YA, class GA, class VA>
 __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forcing& fc, double T,
@@ -299,25 +302,23 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
     ks.region = k1.region + st * NVAR;
     double x[NVAR];
     if (st == 0 || rm.newf[st]) {
-      if (st == 0) {
-        gen::fun_pf16_reg(k, y, h2o, m, x);
-      } else {
-        // Ynew = y + sum_j A(s,j) K_j, accumulated in x, stored for the tendency
+      // the stage's state: y for stage 0, y + sum_j A(s,j) K_j after,
+      // accumulated in x and stored in the stage's region, so that one
+      // instance of the tendency serves every stage
 #pragma unroll
-        for (int i = 0; i < NVAR; i++) x[i] = RD(y, i);
-        for (int j = 0; j < st; j++) {
-          VA kj = k1;
-          kj.region = k1.region + j * NVAR;
-          const double a = rm.A[base + j];
+      for (int i = 0; i < NVAR; i++) x[i] = RD(y, i);
+      for (int j = 0; j < st; j++) {
+        VA kj = k1;
+        kj.region = k1.region + j * NVAR;
+        const double a = rm.A[base + j];
 #pragma unroll
-          for (int i = 0; i < NVAR; i++) x[i] = fma(a, RD(kj, i), x[i]);
-        }
-#pragma unroll
-        for (int i = 0; i < NVAR; i++) WR(ks, i, x[i]);
-        launder(ks);
-        if (forced) photo_at(k, fc, T + rm.alpha[st] * H);
-        gen::fun_pf16_reg(k, ks, h2o, m, x);
+        for (int i = 0; i < NVAR; i++) x[i] = fma(a, RD(kj, i), x[i]);
       }
+#pragma unroll
+      for (int i = 0; i < NVAR; i++) WR(ks, i, x[i]);
+      launder(ks);
+      if (forced && st > 0) photo_at(k, fc, T + rm.alpha[st] * H);
+      gen::fun_pf16_reg(k, ks, h2o, m, x);
 #pragma unroll
       for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
       if (st == 0 && forced) {
@@ -325,7 +326,7 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
         const double delta = forcing_delta(T, fc.dt);
         const double rd = 1.0 / delta;
         photo_at(k, fc, T + delta);
-        gen::fun_pf16_reg(k, y, h2o, m, x);
+        gen::fun_pf16_reg(k, ks, h2o, m, x);
         VA f0 = f;
         launder(f0);
 #pragma unroll
@@ -478,7 +479,25 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
       }
     }
     if (!ros3) {
-      err = generic_stages(rm, fc, s.T, k, y, h2o, m, g, f, k1, H, rtol, atol);
+      // one method at a time, the same for every lane that runs it, so that
+      // its coefficients are read through the scalar cache (a wave whose
+      // cells share a method, the usual case, takes one round)
+      bool pending = true;
+      for (;;) {
+        const unsigned long long want = __ballot(pending);
+        if (want == 0ull) break;
+        const int lead = __ffsll((long long)want) - 1;
+        const int meth = __builtin_amdgcn_readfirstlane(__shfl(o.method, lead));
+        // the table index through an opaque copy: inside the branch the
+        // compiler would otherwise address the table with the per-lane
+        // method it knows to be equal
+        int tab = meth;
+        asm volatile("" : "+s"(tab));
+        if (pending && o.method == meth) {
+          err = generic_stages(kRosMethods[tab], fc, s.T, k, y, h2o, m, g, f, k1, H, rtol, atol);
+          pending = false;
+        }
+      }
     } else {
     double x[NVAR];
     // K1 = G^-1 f(y)

@@ -43,6 +43,17 @@ GOOD = """\ts_and_saveexec_b64 s[0:1], vcc
 """
 
 
+# a long-branch relaxation: an execz test with exec untouched before it
+RELAXED = """\tv_fmac_f64_e32 v[34:35], v[26:27], v[26:27]
+\ts_cbranch_execz .LBB0_165
+\ts_getpc_b64 s[98:99]
+\ts_setpc_b64 s[98:99]
+.LBB0_165:
+\tbuffer_load_dwordx2 v[0:1], v246, s[28:31], s0 offen
+\tscratch_store_dwordx2 off, v[0:1], off offset:1180 ; 8-byte Folded Spill
+"""
+
+
 def _scan(tmp_path, text):
     p = tmp_path / "k.s"
     p.write_text(text)
@@ -56,5 +67,6 @@ def test_flags_spill_under_branch_mask(tmp_path):
 
 
 def test_passes_else_blocks_and_restored_joins(tmp_path):
+    assert _scan(tmp_path, RELAXED) == []
     assert _scan(tmp_path, GOOD) == []
     assert C.main([str(tmp_path / "k.s")]) == 0
