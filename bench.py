@@ -201,7 +201,10 @@ def main():
                                        "with a 1 GiB copy (profiles/radm2_persistent_lds160_reg128_1m.md), "
                                        "scaled to this launch's attempts",
                      "flop_per_launch": flop, "kernel_ms": kms,
-                     "hbm_gbs": (traffic_bytes(r) or 0) / (kms * 1e-3) / 1e9},
+                     "hbm_gbs": (traffic_bytes(r) or 0) / (kms * 1e-3) / 1e9,
+                     "hbm_frac": (traffic_bytes(r) or 0) / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                     "hbm_note": "the measured HBM traffic (spilled working set) against "
+                                 "MI355X's 8 TB/s: the bound the kernel runs at"},
     }
     if "grid" in results and "persistent" in results:
         out["persistent_vs_grid"] = (results["grid"]["wall_s"] / results["persistent"]["wall_s"])
