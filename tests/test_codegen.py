@@ -1,0 +1,89 @@
+"""Invariants of the generated HIP code (codegen/gen_radm2.py), checked on
+the text the generator writes: the hot-set tables place every rate
+constant, state and LU entry exactly once, and the load-scheduled fused
+Jacobian/decomposition never reads an LU entry before the statement that
+stores it."""
+import importlib.util
+import json
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "radm2-solver-multi-gpu_amd")
+spec = importlib.util.spec_from_file_location("gen_radm2", os.path.join(PKG, "codegen", "gen_radm2.py"))
+G = importlib.util.module_from_spec(spec)
+spec.loader.exec_module(G)
+MECH = json.load(open(os.path.join(PKG, "mech", "radm2.json")))
+
+
+@pytest.fixture(scope="module")
+def header(tmp_path_factory):
+    p = tmp_path_factory.mktemp("gen") / "radm2_gen.h"
+    G.generate(str(p))
+    return p.read_text()
+
+
+def _array(text, name):
+    mo = re.search(r"%s\[(\d+)\] = \{([^}]*)\};" % re.escape(name), text)
+    assert mo, name
+    vals = [int(x) for x in mo.group(2).split(",")]
+    assert len(vals) == int(mo.group(1))
+    return vals
+
+
+@pytest.mark.parametrize("budget", G.HOT_BUDGETS)
+def test_hot_tables_partition_the_ranking(header, budget):
+    S, R = budget
+    ranked, acc = G.hot_ranking(MECH)
+    sizes = (MECH["nreact"], MECH["nvar"], MECH["lu"]["nnz"])
+    assert len(ranked) == sum(sizes) and len(set(ranked)) == len(ranked)
+    codes = [_array(header, "HOT%d_%d_%s" % (S, R, n)) for n in ("K", "Y", "G")]
+    lds, regs = [], []
+    for a, arr in enumerate(codes):
+        for i, c in enumerate(arr):
+            if c >= 0:
+                lds.append((c, (a, i)))
+            elif c <= -2:
+                regs.append((-2 - c, (a, i)))
+    # LDS slots 0..S-1 and registers 0..R-1 each hold exactly one entry ...
+    assert sorted(s for s, _ in lds) == list(range(S))
+    assert sorted(r for r, _ in regs) == list(range(R))
+    # ... the S most accessed in LDS, the next R in registers
+    assert {e for _, e in lds} == set(ranked[:S])
+    assert {e for _, e in regs} == set(ranked[S:S + R])
+    # and nothing outside the tiers is accessed more than what is inside
+    inside = min(acc[e] for e in ranked[:S + R]) if S + R else 0
+    assert all(acc[e] <= inside for e in ranked[S + R:])
+
+
+def test_lu_entries_are_read_after_they_are_stored(header):
+    start = header.index("int jac_decomp_pf16(")
+    body = header[start:header.index("\n}\n", start)]
+    stored = {}
+    assert len(re.findall(r"RD\(g, \d+\)", body)) > 300   # the check is not vacuous
+    for n, ln in enumerate(body.split("\n")):
+        for mo in re.finditer(r"RD\(g, (\d+)\)", ln):
+            q = int(mo.group(1))
+            assert q in stored, "LU entry %d read before its store (line %d)" % (q, n)
+        mo = re.match(r"\s*WR\(g, (\d+),", ln)
+        if mo:
+            q = int(mo.group(1))
+            assert q not in stored, "LU entry %d stored twice" % q
+            stored[q] = n
+    assert sorted(stored) == list(range(MECH["lu"]["nnz"]))
+
+
+def test_workspace_and_hot_accesses_use_constant_indices(header):
+    # the register tier needs compile-time indices: every generated access
+    # to the rate constants, the state or the LU names its element literally
+    for mo in re.finditer(r"\b(RD|WR)\((k|v|g), ([^,)]+)", header):
+        assert re.fullmatch(r"\d+", mo.group(3).strip()), mo.group(0)
+
+
+def test_photolysis_map_matches_the_rate_laws():
+    pj = G.photo_j(MECH)
+    for r, s in enumerate(pj):
+        assert MECH["rates"][r].strip() == "(J[%d])" % s
+    assert sorted(pj) == list(range(21))
