@@ -294,3 +294,31 @@ def test_family_plan_replays_match_eager(solver):
         torch.cuda.synchronize()
         _ = (b.conc * 2.0).sum().item()
     plan.close()
+
+
+def test_kpp_frontend_nonautonomous_without_forcing_is_autonomous(solver):
+    # IPAR[0] = 0 with no jrate_end: J is constant, dF/dT = 0, so the
+    # integration is the autonomous one; Nfun still counts the derivative's
+    # evaluation once per step
+    import radm2_amd
+    from radm2_amd import synth
+    cells = synth.make_cells(128, 90, 100.0, 0.05)
+
+    def batch():
+        d = synth.to_device(cells)
+        return radm2_amd.CellBatch(d["conc"], d["temp"], d["cair"], d["h2o"], d["jrate"],
+                                   rc_n2o5=d["rc_n2o5"])
+
+    out = []
+    for ipar0 in (0, 1):
+        b = batch()
+        RPAR = [0.0] * 20
+        IPAR = [0] * 20
+        IPAR[0], IPAR[1], IPAR[3] = ipar0, 1, 2
+        RPAR[2] = 6.0
+        assert solver.rosenbrock(b, 0.0, 600.0, [1.0], [1e-3], RPAR, IPAR) == 1
+        out.append((b.conc.cpu().numpy(), IPAR))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    i0, i1 = out[0][1], out[1][1]
+    assert i0[12] == i1[12] and i0[13] == i1[13]
+    assert i0[10] == i1[10] + i1[13]
