@@ -643,9 +643,9 @@ def hot_ranking(mech):
     kernels, most accessed first: an LU entry is stored once and read by
     the decomposition of every later row that eliminates with it and by the
     three solves; rate constants and state entries count the loads of the
-    Jacobian/decomposition (once per row) and of the tendency (rate
-    constants twice, the state once), plus the state's three element-wise
-    passes.  Ties keep array order."""
+    Jacobian/decomposition (once per row) and of the tendency (the state
+    once; rate constants twice, weighted 3 as measured best), plus the
+    state's three element-wise passes.  Ties keep array order."""
     import collections
     import re
     lu = mech["lu"]
@@ -664,7 +664,9 @@ def hot_ranking(mech):
         acc[(0 if mo.group(1) == "k" else 1, int(mo.group(2)))] += 1
     for mo in re.finditer(r"RD\((k|v), (\d+)\)", gen_fun_pf(mech, 16)):
         if mo.group(1) == "k":
-            acc[(0, int(mo.group(2)))] += 2
+            # two tendencies per attempt; weighted 3, the measured best
+            # (2: 770.7 ms, 3: 758.9 ms, 4: 773.2 ms at 1M cells)
+            acc[(0, int(mo.group(2)))] += 3
         else:
             acc[(1, int(mo.group(2)))] += 1
     for i in range(59):
