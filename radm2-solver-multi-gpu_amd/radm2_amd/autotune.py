@@ -4,15 +4,15 @@ Axes (all compiled in; see include/radm2.h):
   schedule   one-shot grid (lockstep blocks of 64..1024 cells) or
              persistent work stealing (blocks of 64 or 256)
   placement  working arrays compiler-managed (VGPR/AGPR, spills to
-             scratch) or an explicit wave-interleaved HBM workspace
+             scratch), an explicit wave-interleaved HBM workspace, or the
+             workspace with the most accessed entries in LDS (160 or 312
+             doubles per lane) and optionally registers (the next 128);
+             the LDS placements run 64-lane blocks only
   waves      register budget: at least 1, 2, 4 or 8 resident waves/SIMD
-  depth      for the workspace placement, the launder interval (rows of the
-             unrolled LU kept in registers before being re-read) is fixed per
-             register budget: 4 rows at 2 waves, 2 rows at 4 waves
   refill     persistent: idle lanes per wave that trigger a refill
 
-An LDS placement of the LU is not an axis: one cell's LU is 659 doubles
-(5.3 KB), a 64-lane wave's is 337 KB, more than a CU's 160 KB of LDS.
+A cell's whole working set (1,110 doubles, 8.9 KB; 568 KB per wave) does not
+fit on chip, so only the most accessed part is placed there.
 
 The search is coordinate descent on a sample of the caller's own cells
 (every candidate integrates the same cells, so the results also double as a
