@@ -14,6 +14,7 @@ Paths timed, each over the same K steps after W warm-up steps:
               timed region; reported as a speed-up over `persistent`, with
               proposals floored at 1e-12 s and, separately, at 1e-6 s;
   end_to_end  one step from pinned host buffers: H2D, solve, D2H.
+Each path reports histograms of the attempts and rejected steps per cell.
 `accuracy` compares the plain and learned end states of the first 16,384
 cells with the GPU FP64 path at rtol 1e-10, atol 1e-3 molec/cm3.
 `roofline` is the FP64 compute roofline of the solver kernel (the only
@@ -36,6 +37,18 @@ sys.path.insert(0, PKG)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# histogram bin edges: attempts (accepted + rejected steps) and rejected
+# steps per cell; bin i counts edges[i] <= x < edges[i+1], the last bin is
+# open-ended
+ATTEMPT_EDGES = (0, 25, 50, 75, 100, 125, 150, 175, 200, 225, 250, 300, 400, 600, 1000)
+REJECT_EDGES = (0, 1, 2, 4, 8, 12, 16, 24, 32, 64)
+
+
+def histogram(x, edges, torch):
+    """Per-bin counts of the integer tensor `x` over `edges`."""
+    bounds = torch.tensor(edges[1:], dtype=torch.int64, device=x.device)
+    idx = torch.bucketize(x.to(torch.int64), bounds, right=True)
+    return {"edges": list(edges), "counts": torch.bincount(idx, minlength=len(edges)).cpu().tolist()}
 
 
 def parse():
@@ -87,7 +100,9 @@ def run_path(solver, batch, conc0, sched, args, torch, step=None):
     return dict(wall_s=wall, kernel_ms=kernel_ms, nacc=int(nacc.sum()), nrej=int(nrej.sum()),
                 nstep_max=int(batch.nstep.max()), nstep_mean=float(batch.nstep.double().mean()),
                 nrej_mean=float(batch.nrej.double().mean()),
-                ok=int((st == radm2_amd.CELL_OK).sum()))
+                ok=int((st == radm2_amd.CELL_OK).sum()),
+                attempts_hist=histogram(batch.nstep, ATTEMPT_EDGES, torch),
+                rejected_hist=histogram(batch.nrej, REJECT_EDGES, torch))
 
 
 def learned_step(solver, batch, args, hlo):
@@ -192,7 +207,9 @@ def main():
         "paths": {k: {"cells_per_s": n / (v["wall_s"] / args.steps),
                       "kernel_ms_median": float(np.median(v["kernel_ms"])),
                       "attempts_mean": v["nstep_mean"], "attempts_max": v["nstep_max"],
-                      "cells_ok": v["ok"]} for k, v in results.items()},
+                      "rejected_mean": v["nrej_mean"], "cells_ok": v["ok"],
+                      "attempts_hist": v["attempts_hist"], "rejected_hist": v["rejected_hist"]}
+                  for k, v in results.items()},
         "roofline": {"bound": "mfma", "compute": "FP64 VALU (no MFMA on this path); "
                      "MI355X FP64 vector peak = 78.6 TFLOP/s",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
