@@ -14,6 +14,8 @@ Paths timed, each over the same K steps after W warm-up steps:
               timed region; reported as a speed-up over `persistent`, with
               proposals floored at 1e-12 s and, separately, at 1e-6 s;
   end_to_end  one step from pinned host buffers: H2D, solve, D2H.
+Also reported: `family` (KPP's other methods and the forced Ros3 on the same
+cells) and `batched` (the 64 uneven domains as one HIP-graph launch).
 Each path reports histograms of the attempts and rejected steps per cell.
 `accuracy` compares the plain and learned end states of the first 16,384
 cells with the GPU FP64 path at rtol 1e-10, atol 1e-3 molec/cm3.
@@ -238,6 +240,7 @@ def main():
     if not args.no_extra:
         out["end_to_end"] = end_to_end(solver, cells, args, torch)
         out["family"] = family(solver, batch, conc0, args, torch)
+        out["batched"] = batched(solver, args, torch)
     if na > 0 and not args.no_extra:
         out["accuracy"] = accuracy(solver, dev, conc0, ends, na, args, torch)
     if not args.no_cpu:
@@ -272,6 +275,44 @@ def family(solver, batch, conc0, args, torch):
             name, out[name]["kernel_ms"], out[name]["attempts_mean"], out[name]["cells_ok"]),
             file=sys.stderr, flush=True)
     batch.jrate_end = None
+    return out
+
+
+def batched(solver, args, torch):
+    """The 64-domain workload (synth.make_domains(): log-uniform 1k-200k
+    cells per domain, per-domain dt in {60, 300, 600, 3600} s and rtol in
+    {1e-2, 1e-3, 1e-4}) captured as one HIP-graph plan: one warm-up launch,
+    then one timed (the graph's own events: counter reset + solver)."""
+    import radm2_amd
+    from radm2_amd import synth
+    doms = synth.make_domains()
+    batches, conc0 = [], []
+    for c, _, _, _ in doms:
+        d = synth.to_device(c)
+        conc0.append(d["conc"].clone())
+        batches.append(radm2_amd.CellBatch(d["conc"], d["temp"], d["cair"], d["h2o"], d["jrate"],
+                                           rc_n2o5=d["rc_n2o5"]))
+    total = sum(b.ncells for b in batches)
+    solver.set_schedule(radm2_amd.SCHED_PERSISTENT, args.block, args.refill)
+    plan = solver.plan([b.domain(dt, rtol, atol) for b, (_, dt, rtol, atol) in zip(batches, doms)])
+    try:
+        for _ in range(2):
+            for b, c0 in zip(batches, conc0):
+                b.conc.copy_(c0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            plan.launch()
+            solver.sync()
+            wall = time.perf_counter() - t0
+    finally:
+        plan.close()
+    ms = solver.elapsed_ms()
+    attempts = sum(int(b.nstep.to(torch.int64).sum()) for b in batches)
+    out = {"domains": len(doms), "cells": total, "graph_ms": ms, "wall_ms": 1000.0 * wall,
+           "cells_per_s": total / (ms * 1e-3), "attempts_mean": attempts / total,
+           "cells_ok": sum(int((b.status == radm2_amd.CELL_OK).sum()) for b in batches)}
+    print("[bench] batched: %d domains, %d cells, graph %.1f ms, attempts mean %.1f, ok %d" % (
+        len(doms), total, ms, out["attempts_mean"], out["cells_ok"]), file=sys.stderr, flush=True)
     return out
 
 
