@@ -66,7 +66,7 @@ def parse():
     ap.add_argument("--acc-cells", type=int, default=16384,
                     help="cells checked against the rtol-1e-10 GPU solve (0 = skip)")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the end-to-end and accuracy legs (profiling runs)")
+                    help="skip the end-to-end, family, batched and accuracy legs (profiling runs)")
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
@@ -133,7 +133,7 @@ def _net():
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported "
+        print("bench.py measures one GPU; --gpus %d is not supported "
               "(cells are independent: the path would shard by contiguous slices of "
               "cells, one per GPU)" % args.gpus, file=sys.stderr)
         sys.exit(2)

@@ -2,10 +2,10 @@
 
 Counting rules: a multiply, add or divide is 1 FP64 flop, an fma is 2, a
 transcendental (exp, pow, log10, sqrt) is 1.  The counts are the minimum
-work of the reference's algorithm (mgpu.cu CS3 step), not of this build's
+work of the reference's algorithm (mgpu.cu CS3 step), not of the kernels'
 schedule: one Fun and one Jacobian per time step (accepted attempt), and per
 attempt one matrix preparation, one LU decomposition, one more Fun, three
-solves and the stage/combination/error vector work.  This build recomputes
+solves and the stage/combination/error vector work.  The kernels recompute
 Fun(y) and the Jacobian on rejected attempts instead of storing them; that
 extra work is not counted.
 """

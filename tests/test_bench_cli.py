@@ -1,6 +1,6 @@
-"""bench.py on the CPU: more than one GPU is refused with a message (this
-build measures one GPU) before anything touches a device, and the per-cell
-step histograms bin as a linear scan over the edges does."""
+"""bench.py on the CPU: more than one GPU is refused with a message (the
+benchmark measures one GPU) before anything touches a device, and the
+per-cell step histograms bin as a linear scan over the edges does."""
 import os
 import subprocess
 import sys
