@@ -437,7 +437,13 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
   constexpr int RF = R > 0 ? 16 * R : 0;
   double H = s.H;
   const bool forced = FAMILY && fc.j1 != nullptr;
-  const bool ros3 = !FAMILY || (o.method == 2 && !forced);
+  // forced Ros3 runs the fused stages with the forcing terms when every
+  // active lane of the wave is a forced Ros3 cell: the condition is then
+  // wave-uniform, so its branches move no lane mask (a mixed wave sends its
+  // forced cells through the general stage loop, with identical results)
+  bool fused_forced = false;
+  if constexpr (FAMILY) fused_forced = __ballot(forced && o.method == 2) == __ballot(true);
+  const bool ros3 = !FAMILY || (o.method == 2 && (!forced || fused_forced));
   // the family reads its scalars from the table for every lane (Ros3's row
   // holds Ros3's values): a per-lane choice between a constant and a load
   // would be a branch whose merged value the compiler may spill under the
@@ -500,24 +506,74 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
       }
     } else {
     double x[NVAR];
-    // K1 = G^-1 f(y)
-    gen::fun_pf16_reg(k, y, h2o, m, x);
+    // forced: dF/dT in the seventh stage region, the stage times and gamma
+    // sums of Ros3, each term in the general stage loop's order
+    VA dfdt = k1;
+    dfdt.region = k1.region + 6 * NVAR;
+    constexpr RosMethod r3 = kRosMethods[2];
+    // K1 = G^-1 f(y)   (forced: + H gamma_1 dF/dT)
+    if constexpr (FAMILY) {
+      // forced: a second pass of the same tendency code at T + delta, F0
+      // parked in f meanwhile
+      const double delta = forcing_delta(s.T, fc.dt);
+#pragma nounroll
+      for (int pass = 0;; pass++) {
+        gen::fun_pf16_reg(k, y, h2o, m, x);
+        if (!fused_forced || pass > 0) break;
+#pragma unroll
+        for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
+        photo_at(k, fc, s.T + delta);
+      }
+      if (fused_forced) {
+        // dF/dT = (f(T + delta, y) - f(T, y)) / delta
+        const double rd = 1.0 / delta;
+        const double hg = H * r3.gamma[0];
+        VA f0 = f;
+        launder(f0);
+#pragma unroll
+        for (int i = 0; i < NVAR; i++) {
+          const double fi = RD(f0, i);
+          const double d = fma(-1.0, fi, x[i]) * rd;
+          WR(dfdt, i, d);
+          x[i] = fma(hg, d, fi);
+        }
+      }
+    } else {
+      gen::fun_pf16_reg(k, y, h2o, m, x);
+    }
     gen::solve_pf16_reg(g, x);
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(k1, i, x[i]);
     // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
+    //   (forced: F2 at T + alpha_2 H, + H gamma_2 dF/dT)
     vmap<16>(k2, [](int, const double* a) { return a[0] + a[1]; }, y, k1);
+    if constexpr (FAMILY) {
+      if (fused_forced) photo_at(k, fc, s.T + r3.alpha[1] * H);
+    }
     gen::fun_pf16_reg(k, k2, h2o, m, x);
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
     xpass<16>([&x, hc21](int i, const double* a) { x[i] = fma(a[0], hc21, x[i]); }, k1);
+    if constexpr (FAMILY) {
+      if (fused_forced) {
+        const double hg = H * r3.gamma[1];
+        xpass<16>([&x, hg](int i, const double* a) { x[i] = fma(hg, a[0], x[i]); }, dfdt);
+      }
+    }
     gen::solve_pf16_reg(g, x);
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(k2, i, x[i]);
     // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
+    //   (forced: + H gamma_3 dF/dT)
     xpass<16>([&x, hc31, hc32](int i, const double* a) {
       x[i] = fma(a[0], hc32, fma(a[1], hc31, a[2]));
     }, k2, k1, f);
+    if constexpr (FAMILY) {
+      if (fused_forced) {
+        const double hg = H * r3.gamma[2];
+        xpass<16>([&x, hg](int i, const double* a) { x[i] = fma(hg, a[0], x[i]); }, dfdt);
+      }
+    }
     gen::solve_pf16_reg(g, x);
     // combination and WRMS error norm with K3 in registers; Ynew kept in f
     xpass<8>([&x, &err, f, rtol, atol](int i, const double* a) {

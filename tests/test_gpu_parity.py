@@ -265,6 +265,52 @@ def test_mixed_forcing_methods_and_retries_in_one_call(solver):
     solver.set_schedule(radm2_amd.SCHED_PERSISTENT)
 
 
+def test_forced_ros3_fused_stages_match_the_general_loop(solver):
+    # forced Ros3 runs the fused Ros3 stages when every active lane of a wave
+    # is a forced Ros3 cell, else the general stage loop.  One forced domain
+    # (whole waves: the fused stages) against the same cells as one-cell
+    # domains alternating with autonomous cells (mixed waves: the general
+    # loop for the forced cells), bit for bit
+    import torch
+    import radm2_amd
+    from radm2_amd import synth
+    n = 96
+    ca, ja = _forced_cells(n, 81)
+    cb = synth.make_cells(n, 82, 100.0, 0.05)
+    keys = ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")
+
+    def batch(c, jend=None):
+        d = synth.to_device(c)
+        return radm2_amd.CellBatch(d["conc"], d["temp"], d["cair"], d["h2o"], d["jrate"],
+                                   rc_n2o5=d["rc_n2o5"],
+                                   jrate_end=None if jend is None else torch.from_numpy(jend).cuda())
+
+    def cell(c, i):
+        return {k: np.ascontiguousarray(c[k][..., i:i + 1]) for k in keys}
+
+    o = radm2_amd.Opts()
+    for sched in (radm2_amd.SCHED_PERSISTENT, radm2_amd.SCHED_GRID):
+        solver.set_schedule(sched)
+        whole = batch(ca, ja)
+        solver.integrate(whole, 600.0, 1e-3, 1.0, opts=o)
+        singles, others, doms = [], [], []   # every batch held for the call
+        for i in range(n):
+            a = batch(cell(ca, i), np.ascontiguousarray(ja[:, i:i + 1]))
+            b = batch(cell(cb, i))
+            singles.append(a)
+            others.append(b)
+            doms += [a.domain(600.0, 1e-3, 1.0, opts=o), b.domain(600.0, 1e-3, 1.0, opts=o)]
+        solver.integrate_batched(doms)
+        solver.sync()
+        assert (whole.status.cpu().numpy() == 1).all()
+        assert all((x.status.cpu().numpy() == 1).all() for x in others)
+        for key in ("conc", "nstep", "nacc", "nrej", "status", "hexit"):
+            got = np.concatenate([getattr(a, key).cpu().numpy() for a in singles], axis=-1)
+            np.testing.assert_array_equal(got, getattr(whole, key).cpu().numpy(),
+                                          err_msg="%s sched %d" % (key, sched))
+    solver.set_schedule(radm2_amd.SCHED_PERSISTENT)
+
+
 def test_family_plan_replays_match_eager(solver):
     # a captured plan of a forced Rodas3 domain replays bit-identically to the
     # eager call, with eager work between the replays
