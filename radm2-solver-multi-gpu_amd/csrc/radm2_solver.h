@@ -271,7 +271,8 @@ __device__ __forceinline__ double forcing_delta(double T, double dt) {
 }
 
 // Stages of one attempt of KPP's generic Rosenbrock scheme on workspace
-// views (the methods other than Ros3, and every method when forced): stage
+// views (the methods other than Ros3 and, when forced, Ros3 in a wave whose
+// other cells are not all forced Ros3): stage
 // s's vector lives in the workspace region k1.region + s*NVAR, dF/dT in
 // region 6, the latest new F in f.  The reference's commented stage loop
 // (mgpu.cu:3595-3625) in KPP's general form: F = f(T, y) for stage 0,
