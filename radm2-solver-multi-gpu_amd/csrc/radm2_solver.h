@@ -241,6 +241,41 @@ __device__ __forceinline__ void vmap(OA out, Fn fn, const IA&... in) {
   }
 }
 
+// fn(i, {in0[i], in1[i], ..}) for i = 0, 1, .., 58 in order, the inputs
+// streamed CH at a time with the next chunk's loads issued before this
+// chunk's work (fenced, so the compiler keeps them apart); fn may store to
+// views that none of the inputs alias.
+template <int CH, class Fn, class... IA>
+__device__ __forceinline__ void xpass(Fn fn, const IA&... in) {
+  constexpr int N = sizeof...(IA);
+  constexpr int NCH = (NVAR + CH - 1) / CH;
+  double buf[2][N][CH];
+  {
+    int t = 0;
+    (vload<CH>(buf[0][t++], in, 0), ...);
+  }
+#pragma unroll
+  for (int c = 0; c < NCH; c++) {
+    const int cur = c & 1;
+    if (c + 1 < NCH) {
+      int t = 0;
+      (vload<CH>(buf[cur ^ 1][t++], in, c + 1), ...);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < CH; j++) {
+      const int i = c * CH + j;
+      if (i < NVAR) {
+        double v[N];
+#pragma unroll
+        for (int t = 0; t < N; t++) v[t] = buf[cur][t][j];
+        fn(i, v);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // Photolysis forcing of a non-autonomous integration: J(t) = j0 + (t/dt)
 // (j1 - j0) for this cell (j0, j1 at its column, stride ld); j1 null: the
 // integration is autonomous.
@@ -312,8 +347,7 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
         VA kj = k1;
         kj.region = k1.region + j * NVAR;
         const double a = rm.A[base + j];
-#pragma unroll
-        for (int i = 0; i < NVAR; i++) x[i] = fma(a, RD(kj, i), x[i]);
+        xpass<16>([&x, a](int i, const double* v) { x[i] = fma(a, v[0], x[i]); }, kj);
       }
 #pragma unroll
       for (int i = 0; i < NVAR; i++) WR(ks, i, x[i]);
@@ -348,15 +382,13 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
       VA kj = k1;
       kj.region = k1.region + j * NVAR;
       const double c = rm.C[base + j] / H;
-#pragma unroll
-      for (int i = 0; i < NVAR; i++) x[i] = fma(RD(kj, i), c, x[i]);
+      xpass<16>([&x, c](int i, const double* v) { x[i] = fma(v[0], c, x[i]); }, kj);
     }
     if (forced && rm.gamma[st] != 0.0) {
       const double hg = H * rm.gamma[st];
       VA d = dfdt;
       launder(d);
-#pragma unroll
-      for (int i = 0; i < NVAR; i++) x[i] = fma(hg, RD(d, i), x[i]);
+      xpass<16>([&x, hg](int i, const double* v) { x[i] = fma(hg, v[0], x[i]); }, d);
     }
     gen::solve_pf16_reg(g, x);
 #pragma unroll
@@ -364,60 +396,50 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
   }
   launder(k1);
   double err = 0.0;
+  // combination and error norm, CH species at a time so that each stage's
+  // loads for the chunk are issued together; per species the sums still run
+  // over the stages in order, and err over the species in order
+  constexpr int CH = 8;
 #pragma unroll
-  for (int i = 0; i < NVAR; i++) {
-    const double a0 = RD(k1, i);
-    double sm = rm.M[0] * a0, e = rm.E[0] * a0;
+  for (int c = 0; c < NVAR; c += CH) {
+    double sm[CH], e[CH];
+#pragma unroll
+    for (int t = 0; t < CH; t++) {
+      if (c + t < NVAR) {
+        const double a0 = RD(k1, c + t);
+        sm[t] = rm.M[0] * a0;
+        e[t] = rm.E[0] * a0;
+      }
+    }
     for (int j = 1; j < S; j++) {
       VA kj = k1;
       kj.region = k1.region + j * NVAR;
-      const double aj = RD(kj, i);
-      sm = fma(rm.M[j], aj, sm);
-      e = fma(rm.E[j], aj, e);
-    }
-    const double yi = RD(y, i);
-    const double yn = yi + sm;
-    const double ymax = dmax_ref(fabs(yi), fabs(yn));
-    const double q = e / fma(rtol, ymax, atol);
-    err = fma(q, q, err);
-    WR(f, i, yn);
-  }
-  return err;
-}
-
-// fn(i, {in0[i], in1[i], ..}) for i = 0, 1, .., 58 in order, the inputs
-// streamed CH at a time with the next chunk's loads issued before this
-// chunk's work (fenced, so the compiler keeps them apart); fn may store to
-// views that none of the inputs alias.
-template <int CH, class Fn, class... IA>
-__device__ __forceinline__ void xpass(Fn fn, const IA&... in) {
-  constexpr int N = sizeof...(IA);
-  constexpr int NCH = (NVAR + CH - 1) / CH;
-  double buf[2][N][CH];
-  {
-    int t = 0;
-    (vload<CH>(buf[0][t++], in, 0), ...);
-  }
+      double aj[CH];
 #pragma unroll
-  for (int c = 0; c < NCH; c++) {
-    const int cur = c & 1;
-    if (c + 1 < NCH) {
-      int t = 0;
-      (vload<CH>(buf[cur ^ 1][t++], in, c + 1), ...);
-    }
-    __builtin_amdgcn_sched_barrier(0);
+      for (int t = 0; t < CH; t++)
+        if (c + t < NVAR) aj[t] = RD(kj, c + t);
 #pragma unroll
-    for (int j = 0; j < CH; j++) {
-      const int i = c * CH + j;
-      if (i < NVAR) {
-        double v[N];
-#pragma unroll
-        for (int t = 0; t < N; t++) v[t] = buf[cur][t][j];
-        fn(i, v);
+      for (int t = 0; t < CH; t++) {
+        if (c + t < NVAR) {
+          sm[t] = fma(rm.M[j], aj[t], sm[t]);
+          e[t] = fma(rm.E[j], aj[t], e[t]);
+        }
       }
     }
-    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < CH; t++) {
+      if (c + t < NVAR) {
+        const int i = c + t;
+        const double yi = RD(y, i);
+        const double yn = yi + sm[t];
+        const double ymax = dmax_ref(fabs(yi), fabs(yn));
+        const double q = e[t] / fma(rtol, ymax, atol);
+        err = fma(q, q, err);
+        WR(f, i, yn);
+      }
+    }
   }
+  return err;
 }
 
 // One attempt for a running cell whose attempt top has been applied.
