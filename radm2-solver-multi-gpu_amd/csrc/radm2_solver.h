@@ -399,7 +399,7 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
   // combination and error norm, CH species at a time so that each stage's
   // loads for the chunk are issued together; per species the sums still run
   // over the stages in order, and err over the species in order
-  constexpr int CH = 8;
+  constexpr int CH = 4;
 #pragma unroll
   for (int c = 0; c < NVAR; c += CH) {
     double sm[CH], e[CH];
