@@ -174,6 +174,13 @@ int radm2_last_elapsed_ms(radm2_ctx *ctx, float *ms);
 int radm2_enable_stats(radm2_ctx *ctx, int on);
 int radm2_read_stats(radm2_ctx *ctx, uint64_t out[4]);
 
+/* Measured attainable FP64 throughput of the context's device for the
+ * roofline report: eight independent FMA chains per lane in registers, the
+ * best of three timed launches after a warm-up, in TFLOP/s.  Synchronizes
+ * the context's stream; allocates and frees a 4 MB buffer.  No reference
+ * counterpart: a measurement helper. */
+int radm2_measure_fp64_peak(radm2_ctx *ctx, double *fp64_tflops);
+
 /* KPP option check without running: returns 0, the IERR code -1..-5 of
  * Rosenbrock() (mgpu.cu:3715-3811), or RADM2_E_ARG for a negative
  * sing_retries. */

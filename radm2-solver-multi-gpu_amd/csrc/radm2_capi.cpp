@@ -498,6 +498,14 @@ int radm2_sync(radm2_ctx* c) {
   return RADM2_OK;
 }
 
+int radm2_measure_fp64_peak(radm2_ctx* c, double* fp64_tflops) {
+  if (!c || !fp64_tflops) return RADM2_E_ARG;
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(radm2::measure_fp64_peak(c->stream, c->cu_count, fp64_tflops));
+  return RADM2_OK;
+}
+
 int radm2_last_elapsed_ms(radm2_ctx* c, float* ms) {
   if (!c || !ms) return RADM2_E_ARG;
   if (!c->timed) {

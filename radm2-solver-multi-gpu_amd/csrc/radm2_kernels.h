@@ -116,5 +116,6 @@ constexpr int kNumVariants = 11;
 const KernelVariant* kernel_variants();
 
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
+hipError_t measure_fp64_peak(hipStream_t st, int cus, double* fp64_tflops);
 
 }  // namespace radm2

@@ -99,6 +99,7 @@ _EXPORTS = [
     ("radm2_last_elapsed_ms", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
     ("radm2_enable_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("radm2_read_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+    ("radm2_measure_fp64_peak", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     ("radm2_check_opts", ctypes.c_int,
      [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.POINTER(Opts)]),
     ("radm2_rosenbrock", ctypes.c_int,
@@ -310,6 +311,12 @@ class Solver:
 
     def enable_stats(self, on=True):
         _check(self.lib.radm2_enable_stats(self.ctx, 1 if on else 0))
+
+    def measure_fp64_peak(self):
+        """Measured attainable FP64 FMA throughput, TFLOP/s."""
+        t = ctypes.c_double()
+        _check(self.lib.radm2_measure_fp64_peak(self.ctx, ctypes.byref(t)))
+        return t.value
 
     def read_stats(self):
         out = (ctypes.c_uint64 * 4)()

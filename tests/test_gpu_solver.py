@@ -324,3 +324,12 @@ def test_batched_64_uneven_domains_plan_matches_oracle(solver):
         good += int(ok.sum())
     assert good >= 0.98 * nsub * len(doms) and worst >= 0.9, (good, worst)
     plan.close()
+
+
+def test_measured_fp64_peak_is_plausible(solver):
+    # the roofline report's attainable FP64 peak (FMA chains in registers):
+    # positive and not above MI355X's published 78.6 TFLOP/s by more than
+    # measurement noise
+    tflops = solver.measure_fp64_peak()
+    print("attainable FP64 FMA %.1f TFLOP/s" % tflops)
+    assert 10.0 < tflops <= 1.05 * 78.6
