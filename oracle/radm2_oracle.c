@@ -243,12 +243,68 @@ int oracle_resolve(double dt, double rtol, double atol, const oracle_opts_in *in
   return 0;
 }
 
+/* ---- step-size control (acceptStep, mgpu.cu:3238-3260) -------------------
+ * err_sum: the sum over species of the squared scaled errors.  Err =
+ * sqrt(err_sum/NVAR), Fac = min(FacMax, max(FacMin, FacSafe/Err^(1/ELO))),
+ * Hnew = H*Fac.  Accepted when Err <= 1 or H <= Hmin: T += H, Hnew capped at
+ * H after a rejection, both flags cleared.  Rejected: Hnew = H*FacRej after
+ * two rejections in a row, RejectMoreH takes RejectLastH, RejectLastH set.
+ * H takes Hnew either way.  Returns 1 when accepted, 0 when rejected. */
+int oracle_step_control(double err_sum, double elo, const oracle_opts *o, double *H, double *T,
+                        int32_t *rej_last, int32_t *rej_more) {
+  const double err = sqrt(err_sum / (double)O_NVAR);
+  const double fac = O_MIN(o->facmax, O_MAX(o->facmin, o->facsafe / pow(err, 1.0 / elo)));
+  double Hnew = *H * fac;
+  if (err <= 1.0 || *H <= o->hmin) {
+    *T = *T + *H;
+    if (*rej_last) Hnew = O_MIN(Hnew, *H);
+    *rej_last = 0;
+    *rej_more = 0;
+    *H = Hnew;
+    return 1;
+  }
+  if (*rej_more) Hnew = *H * o->facrej;
+  *rej_more = *rej_last;
+  *rej_last = 1;
+  *H = Hnew;
+  return 0;
+}
+
+/* ---- Ros3 stages of one attempt (mgpu.cu:3595-3625, combination 3223-3237) -
+ * G: the decomposed -J + 1/(H gamma1) I; F0 = f(y).  Writes Ynew = y + sum
+ * M_j K_j into yn and, when yerr is not NULL, sum E_j K_j into yerr;
+ * returns the sum of squared scaled errors. */
+static double o_ros3_stages(const double *y, const double *F0, const double *k, double h2o,
+                            double m, double H, double rtol, double atol, const double *G,
+                            double *yn, double *yerr) {
+  double F2[O_NVAR], K1[O_NVAR], K2[O_NVAR], K3[O_NVAR];
+  for (int i = 0; i < O_NVAR; i++) K1[i] = F0[i];
+  oracle_solve(G, K1, 0);
+  for (int i = 0; i < O_NVAR; i++) yn[i] = y[i] + K1[i];
+  oracle_fun(yn, h2o, m, k, F2);
+  const double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
+  for (int i = 0; i < O_NVAR; i++) K2[i] = fma(K1[i], hc21, F2[i]);
+  oracle_solve(G, K2, 0);
+  for (int i = 0; i < O_NVAR; i++) K3[i] = fma(K2[i], hc32, fma(K1[i], hc31, F2[i]));
+  oracle_solve(G, K3, 0);
+  double err = 0.0;
+  for (int i = 0; i < O_NVAR; i++) {
+    double s = fma(ROS_M3, K3[i], fma(ROS_M2, K2[i], ROS_M1 * K1[i]));
+    yn[i] = y[i] + s;
+    double e = fma(ROS_E3, K3[i], fma(ROS_E2, K2[i], ROS_E1 * K1[i]));
+    if (yerr) yerr[i] = e;
+    double ymax = O_MAX(fabs(y[i]), fabs(yn[i]));
+    double q = e / fma(rtol, ymax, atol);
+    err = fma(q, q, err);
+  }
+  return err;
+}
+
 /* ---- one cell (CS3: mgpu.cu:3475-3625 with acceptStep 3223-3260) ---------- */
 int oracle_ros3_cell(double *y, const double *k, double h2o, double m, double dt, double rtol,
                      double atol, const oracle_opts *o, double h0, int32_t *nstep,
                      int32_t *nacc, int32_t *nrej, double *hexit) {
-  double G[O_LU_NNZ], F0[O_NVAR], F2[O_NVAR], K1[O_NVAR], K2[O_NVAR], K3[O_NVAR];
-  double yn[O_NVAR];
+  double G[O_LU_NNZ], F0[O_NVAR], yn[O_NVAR];
   const double Tend = dt, rnd = o->roundoff;
   double T = 0.0;
   double H = O_MIN(o->hstart, o->hmax);
@@ -257,7 +313,8 @@ int oracle_ros3_cell(double *y, const double *k, double h2o, double m, double dt
     H = O_MIN(O_MIN(h0, dt), o->hmax);
     if (fabs(H) <= 10.0 * rnd) H = DELTA_MIN;
   }
-  int rejLast = 0, rejMore = 0, status = 1;
+  int32_t rejLast = 0, rejMore = 0;
+  int status = 1;
   int32_t ns = 0, na = 0, nr = 0;
   while ((T - Tend) + rnd <= 0.0) {
     int first = 1;
@@ -271,42 +328,13 @@ int oracle_ros3_cell(double *y, const double *k, double h2o, double m, double dt
       const double ghinv = 1.0 / (H * ROS_GAMMA1);
       oracle_jac_prep(y, h2o, m, k, ghinv, G);
       if (oracle_decomp(G, 0) != 0) { status = -8; goto done; }
-      for (int i = 0; i < O_NVAR; i++) K1[i] = F0[i];
-      oracle_solve(G, K1, 0);
-      for (int i = 0; i < O_NVAR; i++) yn[i] = y[i] + K1[i];
-      oracle_fun(yn, h2o, m, k, F2);
-      const double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
-      for (int i = 0; i < O_NVAR; i++) K2[i] = fma(K1[i], hc21, F2[i]);
-      oracle_solve(G, K2, 0);
-      for (int i = 0; i < O_NVAR; i++) K3[i] = fma(K2[i], hc32, fma(K1[i], hc31, F2[i]));
-      oracle_solve(G, K3, 0);
-      double err = 0.0;
-      for (int i = 0; i < O_NVAR; i++) {
-        double s = fma(ROS_M3, K3[i], fma(ROS_M2, K2[i], ROS_M1 * K1[i]));
-        yn[i] = y[i] + s;
-        double e = fma(ROS_E3, K3[i], fma(ROS_E2, K2[i], ROS_E1 * K1[i]));
-        double ymax = O_MAX(fabs(y[i]), fabs(yn[i]));
-        double q = e / fma(rtol, ymax, atol);
-        err = fma(q, q, err);
-      }
-      err = sqrt(err / (double)O_NVAR);
-      double fac = O_MIN(o->facmax, O_MAX(o->facmin, o->facsafe / pow(err, 1.0 / ROS_ELO)));
-      double Hnew = H * fac;
+      const double err = o_ros3_stages(y, F0, k, h2o, m, H, rtol, atol, G, yn, NULL);
       ns++;
-      if (err <= 1.0 || H <= o->hmin) {
+      if (oracle_step_control(err, ROS_ELO, o, &H, &T, &rejLast, &rejMore)) {
         for (int i = 0; i < O_NVAR; i++) y[i] = yn[i];
-        T = T + H;
-        if (rejLast) Hnew = O_MIN(Hnew, H);
-        rejLast = 0;
-        rejMore = 0;
-        H = Hnew;
         na++;
         break;
       }
-      if (rejMore) Hnew = H * o->facrej;
-      rejMore = rejLast;
-      rejLast = 1;
-      H = Hnew;
       nr++;
     }
   }
@@ -468,12 +496,63 @@ void oracle_photo_at(double *k, const double *j0, const double *j1, double t, do
   }
 }
 
+/* Stages of one attempt of the generic scheme with method table r: G is the
+ * decomposed -J + 1/(H gamma1) I, F0 = f(T, y), kk the rate constants
+ * (their photolysis entries are moved to T + alpha_s H when j1 is not NULL,
+ * and dFdT enters each stage then).  Writes Ynew into yn and, when yerr is
+ * not NULL, sum E_j K_j into yerr; returns the sum of squared scaled errors. */
+static double o_ros_stages(const o_method *r, const double *y, const double *F0, double *kk,
+                           const double *j0, const double *j1, const double *dFdT, double T,
+                           double dt, double h2o, double m, double H, double rtol, double atol,
+                           const double *G, double *yn, double *yerr) {
+  double F[O_NVAR], K[6][O_NVAR];
+  const int nonauto = j1 != NULL;
+  for (int st = 0; st < r->S; st++) {
+    const int base = st * (st - 1) / 2;
+    if (st == 0) {
+      memcpy(F, F0, sizeof F);
+    } else if (r->newf[st]) {
+      for (int i = 0; i < O_NVAR; i++) {
+        double v = y[i];
+        for (int j = 0; j < st; j++) v = fma(r->A[base + j], K[j][i], v);
+        yn[i] = v;
+      }
+      if (nonauto) oracle_photo_at(kk, j0, j1, T + r->alpha[st] * H, dt);
+      oracle_fun(yn, h2o, m, kk, F);
+    }
+    for (int i = 0; i < O_NVAR; i++) {
+      double v = F[i];
+      for (int j = 0; j < st; j++) v = fma(K[j][i], r->C[base + j] / H, v);
+      K[st][i] = v;
+    }
+    if (nonauto && r->gamma[st] != 0.0) {
+      const double hg = H * r->gamma[st];
+      for (int i = 0; i < O_NVAR; i++) K[st][i] = fma(hg, dFdT[i], K[st][i]);
+    }
+    oracle_solve(G, K[st], 0);
+  }
+  double err = 0.0;
+  for (int i = 0; i < O_NVAR; i++) {
+    double sm = r->M[0] * K[0][i], e = r->E[0] * K[0][i];
+    for (int j = 1; j < r->S; j++) {
+      sm = fma(r->M[j], K[j][i], sm);
+      e = fma(r->E[j], K[j][i], e);
+    }
+    yn[i] = y[i] + sm;
+    if (yerr) yerr[i] = e;
+    double ymax = O_MAX(fabs(y[i]), fabs(yn[i]));
+    double q = e / fma(rtol, ymax, atol);
+    err = fma(q, q, err);
+  }
+  return err;
+}
+
 int oracle_ros_cell(int method, double *y, const double *k, const double *j0, const double *j1,
                     double h2o, double m, double dt, double rtol, double atol,
                     const oracle_opts *o, double h0, int32_t *nstep, int32_t *nacc,
                     int32_t *nrej, double *hexit) {
   const o_method r = o_method_table(method);
-  double G[O_LU_NNZ], F0[O_NVAR], F[O_NVAR], K[6][O_NVAR], yn[O_NVAR], dFdT[O_NVAR];
+  double G[O_LU_NNZ], F0[O_NVAR], yn[O_NVAR], dFdT[O_NVAR];
   double kk[O_NREACT];
   const int nonauto = j1 != NULL;
   memcpy(kk, k, sizeof kk);
@@ -485,7 +564,8 @@ int oracle_ros_cell(int method, double *y, const double *k, const double *j0, co
     H = O_MIN(O_MIN(h0, dt), o->hmax);
     if (fabs(H) <= 10.0 * rnd) H = DELTA_MIN;
   }
-  int rejLast = 0, rejMore = 0, status = 1;
+  int32_t rejLast = 0, rejMore = 0;
+  int status = 1;
   int32_t ns = 0, na = 0, nr = 0;
   while ((T - Tend) + rnd <= 0.0) {
     int first = 1;
@@ -512,60 +592,14 @@ int oracle_ros_cell(int method, double *y, const double *k, const double *j0, co
         if (nsc >= o->sing_retries) { status = -8; goto done; }
         H = H * 0.5;
       }
-      for (int st = 0; st < r.S; st++) {
-        const int base = st * (st - 1) / 2;
-        if (st == 0) {
-          memcpy(F, F0, sizeof F);
-        } else if (r.newf[st]) {
-          for (int i = 0; i < O_NVAR; i++) {
-            double v = y[i];
-            for (int j = 0; j < st; j++) v = fma(r.A[base + j], K[j][i], v);
-            yn[i] = v;
-          }
-          if (nonauto) oracle_photo_at(kk, j0, j1, T + r.alpha[st] * H, dt);
-          oracle_fun(yn, h2o, m, kk, F);
-        }
-        for (int i = 0; i < O_NVAR; i++) {
-          double v = F[i];
-          for (int j = 0; j < st; j++) v = fma(K[j][i], r.C[base + j] / H, v);
-          K[st][i] = v;
-        }
-        if (nonauto && r.gamma[st] != 0.0) {
-          const double hg = H * r.gamma[st];
-          for (int i = 0; i < O_NVAR; i++) K[st][i] = fma(hg, dFdT[i], K[st][i]);
-        }
-        oracle_solve(G, K[st], 0);
-      }
-      double err = 0.0;
-      for (int i = 0; i < O_NVAR; i++) {
-        double sm = r.M[0] * K[0][i], e = r.E[0] * K[0][i];
-        for (int j = 1; j < r.S; j++) {
-          sm = fma(r.M[j], K[j][i], sm);
-          e = fma(r.E[j], K[j][i], e);
-        }
-        yn[i] = y[i] + sm;
-        double ymax = O_MAX(fabs(y[i]), fabs(yn[i]));
-        double q = e / fma(rtol, ymax, atol);
-        err = fma(q, q, err);
-      }
-      err = sqrt(err / (double)O_NVAR);
-      double fac = O_MIN(o->facmax, O_MAX(o->facmin, o->facsafe / pow(err, 1.0 / r.elo)));
-      double Hnew = H * fac;
+      const double err = o_ros_stages(&r, y, F0, kk, j0, j1, dFdT, T, dt, h2o, m, H, rtol, atol,
+                                      G, yn, NULL);
       ns++;
-      if (err <= 1.0 || H <= o->hmin) {
+      if (oracle_step_control(err, r.elo, o, &H, &T, &rejLast, &rejMore)) {
         for (int i = 0; i < O_NVAR; i++) y[i] = yn[i];
-        T = T + H;
-        if (rejLast) Hnew = O_MIN(Hnew, H);
-        rejLast = 0;
-        rejMore = 0;
-        H = Hnew;
         na++;
         break;
       }
-      if (rejMore) Hnew = H * o->facrej;
-      rejMore = rejLast;
-      rejLast = 1;
-      H = Hnew;
       nr++;
     }
   }
@@ -575,6 +609,48 @@ done:
   if (nrej) *nrej = nr;
   if (hexit) *hexit = H;
   return status;
+}
+
+/* One autonomous attempt from y with step H (no step control): G = -J(y) +
+ * 1/(H gamma1) I decomposed, F0 = f(y), then the method's stages.  method 2
+ * runs the Ros3 stages oracle_ros3_cell uses, the others the generic loop.
+ * Writes Ynew and sum_j E_j K_j; returns the sum of squared scaled errors,
+ * or -1 when the matrix is singular. */
+double oracle_ros_attempt(int method, const double *y, const double *k, double h2o, double m,
+                          double H, double rtol, double atol, double *ynew, double *yerr) {
+  const o_method r = o_method_table(method);
+  double G[O_LU_NNZ], F0[O_NVAR], kk[O_NREACT];
+  memcpy(kk, k, sizeof kk);
+  oracle_jac_prep(y, h2o, m, kk, 1.0 / (H * r.gamma1), G);
+  if (oracle_decomp(G, 0) != 0) return -1.0;
+  oracle_fun(y, h2o, m, kk, F0);
+  if (method == 2) return o_ros3_stages(y, F0, kk, h2o, m, H, rtol, atol, G, ynew, yerr);
+  return o_ros_stages(&r, y, F0, kk, NULL, NULL, NULL, 0.0, 1.0, h2o, m, H, rtol, atol, G, ynew,
+                      yerr);
+}
+
+/* oracle_ros_attempt through the generic loop with a caller's table (the
+ * layout of oracle_method_coeffs: A[15], C[15], M[6], E[6], newf[6]). */
+double oracle_ros_attempt_tab(int S, const double *A, const double *C, const double *M,
+                              const double *E, const int32_t *newf, double gamma1,
+                              const double *y, const double *k, double h2o, double m, double H,
+                              double rtol, double atol, double *ynew, double *yerr) {
+  o_method r;
+  memset(&r, 0, sizeof r);
+  r.S = S;
+  memcpy(r.A, A, sizeof r.A);
+  memcpy(r.C, C, sizeof r.C);
+  memcpy(r.M, M, sizeof r.M);
+  memcpy(r.E, E, sizeof r.E);
+  for (int i = 0; i < 6; i++) r.newf[i] = newf[i];
+  r.gamma1 = gamma1;
+  double G[O_LU_NNZ], F0[O_NVAR], kk[O_NREACT];
+  memcpy(kk, k, sizeof kk);
+  oracle_jac_prep(y, h2o, m, kk, 1.0 / (H * gamma1), G);
+  if (oracle_decomp(G, 0) != 0) return -1.0;
+  oracle_fun(y, h2o, m, kk, F0);
+  return o_ros_stages(&r, y, F0, kk, NULL, NULL, NULL, 0.0, 1.0, h2o, m, H, rtol, atol, G, ynew,
+                      yerr);
 }
 
 int oracle_integrate(int64_t n, int64_t ld, double *conc, const double *temp,

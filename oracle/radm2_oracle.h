@@ -88,6 +88,21 @@ int oracle_method_stages(int method);
  * g_elo[2] = {gamma, ELO} */
 void oracle_method_coeffs(int method, double *A, double *C, double *M, double *E,
                           double *alpha, double *gamma, double *g_elo);
+/* acceptStep's step-size control (mgpu.cu:3238-3260) on the sum of squared
+ * scaled errors: updates H, T and the two rejection flags; returns 1 when
+ * the attempt is accepted, 0 when rejected. */
+int oracle_step_control(double err_sum, double elo, const oracle_opts *o, double *H, double *T,
+                        int32_t *rej_last, int32_t *rej_more);
+/* one autonomous attempt of method 1..5 from y with step H: Ynew, the error
+ * vector sum_j E_j K_j; returns the sum of squared scaled errors (-1:
+ * singular matrix) */
+double oracle_ros_attempt(int method, const double *y, const double *k, double h2o, double m,
+                          double H, double rtol, double atol, double *ynew, double *yerr);
+/* the same through the generic stage loop with a caller's coefficient table */
+double oracle_ros_attempt_tab(int S, const double *A, const double *C, const double *M,
+                              const double *E, const int32_t *newf, double gamma1,
+                              const double *y, const double *k, double h2o, double m, double H,
+                              double rtol, double atol, double *ynew, double *yerr);
 /* photolysis rate constants at time t of [0, dt] into k */
 void oracle_photo_at(double *k, const double *j0, const double *j1, double t, double dt);
 

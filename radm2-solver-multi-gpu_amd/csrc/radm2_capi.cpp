@@ -38,6 +38,7 @@ struct radm2_ctx {
   int doms_cap = 0;
   unsigned long long* d_counter = nullptr;  // [1]
   unsigned long long* d_stats = nullptr;    // [4]
+  unsigned long long* d_slots = nullptr;    // [kSlotWords] grid block-slot bitmap
   bool stats_on = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_staged = nullptr;
   bool timed = false;
@@ -47,6 +48,7 @@ struct radm2_plan {
   radm2_ctx* ctx = nullptr;
   DomainDesc* d_doms = nullptr;
   unsigned long long* d_counter = nullptr;
+  unsigned long long* d_slots = nullptr;
   double* d_ws = nullptr;
   size_t ws_bytes = 0;
   hipGraph_t graph = nullptr;
@@ -70,6 +72,10 @@ hipError_t& last_hip_error() {
 }
 
 constexpr int kDefaultGridBlock = 64;
+// grid schedule: bitmap words of the block-slot pool (64 slots each) and
+// the slots added to the resident-block count as a margin
+constexpr int kSlotWords = 128;
+constexpr int kSlotMargin = 8;
 constexpr int kDefaultPersistentBlock = 64;
 // Defaults measured on MI355X at 1M cells (the autotuner may override):
 // persistent schedule, explicit workspace placement with its load-scheduled
@@ -200,12 +206,57 @@ int pick_block(const radm2_ctx* c) {
   return c->schedule == RADM2_SCHED_GRID ? kDefaultGridBlock : kDefaultPersistentBlock;
 }
 
+// Resident blocks per CU of a variant's kernel (a schedule's occupancy
+// query); <= 0 records the HIP error and returns RADM2_E_HIP.
+int resident_per_cu(int (*query)(int), int block, int* per_cu) {
+  *per_cu = query(block);
+  if (*per_cu > 0) return 0;
+  last_hip_error() = *per_cu < 0 ? (hipError_t)(-*per_cu) : hipErrorInvalidConfiguration;
+  return RADM2_E_HIP;
+}
+
+// Launch shape of a call: the variant, block size, grid blocks (persistent:
+// resident blocks; grid: blocks of the grid), workspace wave slots and, for
+// the grid on a workspace placement, the block slots of its pool.
+struct Shape {
+  const radm2::KernelVariant* v = nullptr;
+  int block = 0;
+  int64_t blocks = 0, waves = 0, slots = 0;
+};
+
+int launch_shape(const radm2_ctx* c, int64_t total, bool family, Shape* sh) {
+  sh->block = pick_block(c);
+  sh->v = pick_variant(c, family);
+  if (!sh->v) return family ? RADM2_E_UNSUPPORTED : RADM2_E_ARG;
+  const radm2::KernelVariant* v = sh->v;
+  const int block = sh->block;
+  if (v->placement >= radm2::PLACE_HOT160 && block != 64) return RADM2_E_ARG;
+  const int64_t wpb = block / radm2::WAVE;
+  const int64_t need = (total + block - 1) / block;
+  int per_cu = 0;
+  if (c->schedule == RADM2_SCHED_GRID) {
+    sh->blocks = need;
+    if (v->placement >= radm2::PLACE_WS) {
+      if (int rc = resident_per_cu(v->grid_blocks_per_cu, block, &per_cu)) return rc;
+      sh->slots = std::min<int64_t>(need, (int64_t)per_cu * c->cu_count + kSlotMargin);
+      if (sh->slots > 64 * kSlotWords) return RADM2_E_UNSUPPORTED;
+    }
+    sh->waves = sh->slots * wpb;
+  } else {
+    if (int rc = resident_per_cu(v->blocks_per_cu, block, &per_cu)) return rc;
+    sh->blocks = std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->cu_count, need));
+    sh->waves = sh->blocks * wpb;
+  }
+  return 0;
+}
+
 // Launch the kernels for an uploaded table on stream st.  ws/ws_bytes: the
 // workspace to use (grown here when the explicit placement needs more);
 // grow = false for graph capture, where the plan sized it beforehand.
+// slots: the grid schedule's block-slot bitmap (kSlotWords words).
 int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
-            unsigned long long* counter, hipStream_t st, double** ws, size_t* ws_bytes,
-            bool family, bool grow = true) {
+            unsigned long long* counter, unsigned long long* slots, hipStream_t st, double** ws,
+            size_t* ws_bytes, bool family, bool grow = true) {
   LaunchArgs a;
   a.doms = d_doms;
   a.ndom = ndom;
@@ -213,39 +264,32 @@ int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
   a.counter = counter;
   a.stats = c->stats_on ? c->d_stats : nullptr;
   a.refill_min = c->refill_min ? c->refill_min : kDefaultRefillMin;
-  const int block = pick_block(c);
-  const radm2::KernelVariant* v = pick_variant(c, family);
-  if (!v) return family ? RADM2_E_UNSUPPORTED : RADM2_E_ARG;
-  if (v->placement >= radm2::PLACE_HOT160 && block != 64) return RADM2_E_ARG;
-  int64_t blocks = 0, waves = 0;
-  if (c->schedule == RADM2_SCHED_GRID) {
-    // every wave of the launched grid owns a slot, padding waves included
-    waves = (total + block - 1) / block * (block / radm2::WAVE);
-  } else {
-    int per_cu = v->blocks_per_cu(block);
-    if (per_cu <= 0) return RADM2_E_HIP;
-    blocks = (int64_t)per_cu * c->cu_count;
-    const int64_t need = (total + block - 1) / block;
-    blocks = std::max<int64_t>(1, std::min(blocks, need));
-    waves = blocks * (block / radm2::WAVE);
-  }
+  a.slot_bits = slots;
+  a.slot_words = 0;
+  Shape sh;
+  if (int rc = launch_shape(c, total, family, &sh)) return rc;
+  const radm2::KernelVariant* v = sh.v;
   if (v->placement >= radm2::PLACE_WS) {
     if (grow) {
-      if (int rc = ensure_ws(st, ws, ws_bytes, waves, v->ws_doubles)) return rc;
-    } else if (*ws_bytes < (size_t)waves * v->ws_doubles * radm2::WAVE * sizeof(double)) {
+      if (int rc = ensure_ws(st, ws, ws_bytes, sh.waves, v->ws_doubles)) return rc;
+    } else if (*ws_bytes < (size_t)sh.waves * v->ws_doubles * radm2::WAVE * sizeof(double)) {
       return RADM2_E_ARG;
     }
     a.ws = *ws;
-    a.ws_waves = waves;
+    a.ws_waves = sh.waves;
   } else {
     a.ws = nullptr;
     a.ws_waves = 0;
   }
   if (c->schedule == RADM2_SCHED_GRID) {
-    HIP_TRY(v->grid(block, a, st));
+    if (sh.slots > 0) {
+      a.slot_words = (int32_t)((sh.slots + 63) / 64);
+      HIP_TRY(radm2::launch_slot_init(slots, a.slot_words, sh.slots, st));
+    }
+    HIP_TRY(v->grid(sh.block, a, st));
   } else {
     HIP_TRY(radm2::launch_fill_u64(counter, 1, 0ull, st));
-    HIP_TRY(v->persistent(block, (int)blocks, a, st));
+    HIP_TRY(v->persistent(sh.block, (int)sh.blocks, a, st));
   }
   return 0;
 }
@@ -299,6 +343,8 @@ int radm2_create(radm2_ctx** ctx, int device, void* stream) {
   }
   if ((e = hipMalloc(&c->d_counter, sizeof(unsigned long long))) != hipSuccess) return fail(e);
   if ((e = hipMalloc(&c->d_stats, 4 * sizeof(unsigned long long))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&c->d_slots, kSlotWords * sizeof(unsigned long long))) != hipSuccess)
+    return fail(e);
   if ((e = hipMemset(c->d_stats, 0, 4 * sizeof(unsigned long long))) != hipSuccess) return fail(e);
   if ((e = hipEventCreate(&c->ev0)) != hipSuccess) return fail(e);
   if ((e = hipEventCreate(&c->ev1)) != hipSuccess) return fail(e);
@@ -320,6 +366,7 @@ void radm2_destroy(radm2_ctx* c) {
   if (c->h_doms) hipHostFree(c->h_doms);
   if (c->d_counter) hipFree(c->d_counter);
   if (c->d_stats) hipFree(c->d_stats);
+  if (c->d_slots) hipFree(c->d_slots);
   if (c->d_ws) hipFree(c->d_ws);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -375,8 +422,8 @@ int radm2_integrate_batched(radm2_ctx* c, int ndom, const radm2_domain* doms) {
                          hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipEventRecord(c->ev_staged, c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  if (int rc = enqueue(c, c->d_doms, ndom, total, c->d_counter, c->stream, &c->d_ws, &c->ws_bytes,
-                       family))
+  if (int rc = enqueue(c, c->d_doms, ndom, total, c->d_counter, c->d_slots, c->stream, &c->d_ws,
+                       &c->ws_bytes, family))
     return rc;
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   c->timed = true;
@@ -426,39 +473,43 @@ int radm2_plan_create(radm2_ctx* c, int ndom, const radm2_domain* doms, radm2_pl
     radm2_plan_destroy(p);
     return RADM2_E_HIP;
   };
+  // the launch shape is checked before anything is allocated or captured
+  Shape sh;
+  if (int rc = launch_shape(c, total, family, &sh)) {
+    radm2_plan_destroy(p);
+    return rc;
+  }
   hipError_t e;
   if ((e = hipMalloc(&p->d_doms, sizeof(DomainDesc) * ndom)) != hipSuccess) return fail(e);
   if ((e = hipMalloc(&p->d_counter, sizeof(unsigned long long))) != hipSuccess) return fail(e);
+  if ((e = hipMalloc(&p->d_slots, kSlotWords * sizeof(unsigned long long))) != hipSuccess)
+    return fail(e);
   if ((e = hipMemcpy(p->d_doms, tab.data(), sizeof(DomainDesc) * ndom, hipMemcpyHostToDevice)) !=
       hipSuccess)
     return fail(e);
-  // capture on a private stream: counter reset (a fill kernel, no memset
-  // node) followed by the solver launch
+  // capture on a private stream: counter reset or slot-pool reset (fill
+  // kernels, no memset node) followed by the solver launch
   hipStream_t cap;
   if ((e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)) != hipSuccess) return fail(e);
   // size the plan's own workspace before capture (no allocation inside it)
-  {
-    const int block = pick_block(c);
-    const radm2::KernelVariant* v = pick_variant(c, family);
-    int64_t waves = (total + block - 1) / block * (block / radm2::WAVE);
-    if (v && c->schedule == RADM2_SCHED_PERSISTENT) {
-      const int per_cu = v->blocks_per_cu(block);
-      waves = std::min<int64_t>(waves, (int64_t)std::max(per_cu, 1) * c->cu_count * (block / radm2::WAVE));
-    }
-    if (v && v->placement >= radm2::PLACE_WS && ensure_ws(cap, &p->d_ws, &p->ws_bytes, waves, v->ws_doubles) != 0) {
-      hipStreamDestroy(cap);
-      return fail(last_hip_error());
-    }
+  if (sh.v->placement >= radm2::PLACE_WS &&
+      ensure_ws(cap, &p->d_ws, &p->ws_bytes, sh.waves, sh.v->ws_doubles) != 0) {
+    hipStreamDestroy(cap);
+    return fail(last_hip_error());
   }
+  int rc = 0;
   e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
-    int rc = enqueue(c, p->d_doms, ndom, total, p->d_counter, cap, &p->d_ws, &p->ws_bytes, family,
-                     false);
-    hipError_t e2 = hipStreamEndCapture(cap, &p->graph);
-    if (rc != 0 && e2 == hipSuccess) e2 = last_hip_error();
-    e = e2;
+    rc = enqueue(c, p->d_doms, ndom, total, p->d_counter, p->d_slots, cap, &p->d_ws, &p->ws_bytes,
+                 family, false);
+    e = hipStreamEndCapture(cap, &p->graph);
   }
   hipStreamDestroy(cap);
+  if (rc != 0) {
+    // the enqueue's own error (its HIP cause, if any, is already recorded)
+    radm2_plan_destroy(p);
+    return rc;
+  }
   if (e != hipSuccess) return fail(e);
   if ((e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0)) != hipSuccess)
     return fail(e);
@@ -487,6 +538,7 @@ void radm2_plan_destroy(radm2_plan* p) {
   if (p->graph) hipGraphDestroy(p->graph);
   if (p->d_doms) hipFree(p->d_doms);
   if (p->d_counter) hipFree(p->d_counter);
+  if (p->d_slots) hipFree(p->d_slots);
   if (p->d_ws) hipFree(p->d_ws);
   delete p;
 }

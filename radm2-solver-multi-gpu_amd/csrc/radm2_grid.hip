@@ -1,6 +1,9 @@
 // One-shot grid kernel: one lane per cell for the whole integration.
 // A wave runs until its slowest cell finishes (the reference's schedule,
-// mgpu.cu:3504-3641, with the per-cell loop moved onto the lane).
+// mgpu.cu:3504-3641, with the per-cell loop moved onto the lane).  On the
+// workspace placements a block borrows one of the workspace's block slots
+// for its lifetime (acquire_slot/release_slot), so the workspace is sized by
+// the blocks that can be resident, not by the grid.
 #include "radm2_variant.h"
 
 namespace radm2 {
@@ -22,7 +25,13 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
 #if RADM2_PLACE == RADM2_PLACE_PRIVATE
   double kk[NREACT], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
 #else
-  const int64_t wslot = __builtin_amdgcn_readfirstlane((int)(g / WAVE));
+  // thread 0 is the block's first cell, valid whenever the block has one
+  __shared__ int block_slot;
+  if (threadIdx.x == 0) block_slot = acquire_slot(a.slot_bits, a.slot_words, (int)blockIdx.x);
+  __syncthreads();
+  const int slot = block_slot;
+  const int64_t wslot =
+      __builtin_amdgcn_readfirstlane((int)((int64_t)slot * (BLOCK / WAVE) + threadIdx.x / WAVE));
   constexpr int kWsDoubles = RADM2_METHODS ? WS_DOUBLES_FAMILY : WS_DOUBLES_ROS3;
   constexpr int kWsBytes = kWsDoubles * WAVE * 8;
   double* wbase = a.ws + wslot * ((int64_t)kWsDoubles * WAVE);
@@ -69,10 +78,32 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
     }
   }
   if (valid) cell_store(d, p, y, s, s.hfirst, a.stats);
+#if RADM2_PLACE != RADM2_PLACE_PRIVATE
+  // every wave of the block is done with the slot's workspace
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) release_slot(a.slot_bits, slot);
+#endif
 }
 
 }  // namespace RADM2_VARIANT_NS
 using namespace RADM2_VARIANT_NS;
+
+// resident blocks per CU, or minus the HIP error of the occupancy query
+int RADM2_NAME(grid_blocks_per_cu)(int block) {
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (block) {
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_grid<64, false>, 64, 0); break;
+#if RADM2_PLACE < RADM2_PLACE_LDS160
+    case 256: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_grid<256, true>, 256, 0); break;
+    case 512: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_grid<512, true>, 512, 0); break;
+    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_ros3_grid<1024, true>, 1024, 0); break;
+#endif
+    default: break;
+  }
+  return e == hipSuccess ? nb : -(int)e;
+}
 
 hipError_t RADM2_NAME(launch_grid)(int block, const LaunchArgs& a, hipStream_t st) {
   if (a.total <= 0) return hipSuccess;

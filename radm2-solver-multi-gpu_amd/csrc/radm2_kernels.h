@@ -41,6 +41,11 @@ struct LaunchArgs {
   unsigned long long* stats;    // [4] attempts, accepted, rejected, refills (nullable)
   double* ws;                   // explicit workspace (WS placement), ws_doubles per lane
   int64_t ws_waves;             // wave slots the workspace holds
+  // grid schedule on a workspace placement: occupancy bitmap of the
+  // workspace's block slots (BLOCK/64 wave slots each; bit set = taken),
+  // slot_words 64-bit words, bits past the last slot preset
+  unsigned long long* slot_bits;
+  int32_t slot_words;
 };
 
 // Explicit-workspace layout per lane (doubles): rate constants, state,
@@ -85,6 +90,7 @@ constexpr int hot_regs(int placement) { return kHotConfig[placement][1]; }
 // slow Ros3.
 #define RADM2_DECLARE_VARIANT_NAMED(S)                                                   \
   hipError_t launch_grid##S(int block, const LaunchArgs& a, hipStream_t st);             \
+  int grid_blocks_per_cu##S(int block);                                                  \
   hipError_t launch_persistent##S(int block, int blocks, const LaunchArgs& a, hipStream_t st); \
   int persistent_blocks_per_cu##S(int block);
 #define RADM2_DECLARE_VARIANT(P, W, R) RADM2_DECLARE_VARIANT_NAMED(_p##P##_w##W##_r##R)
@@ -111,11 +117,39 @@ struct KernelVariant {
   hipError_t (*grid)(int, const LaunchArgs&, hipStream_t);
   hipError_t (*persistent)(int, int, const LaunchArgs&, hipStream_t);
   int (*blocks_per_cu)(int);
+  int (*grid_blocks_per_cu)(int);
 };
 constexpr int kNumVariants = 11;
 const KernelVariant* kernel_variants();
 
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);
+// clear an occupancy bitmap of `nslots` slots over `nwords` words (the bits
+// past the last slot set)
+hipError_t launch_slot_init(unsigned long long* bits, int nwords, int64_t nslots, hipStream_t st);
+
+// Block-slot pool of the grid schedule: a block takes a free slot of the
+// bitmap when it starts and returns it when its waves are done with the
+// slot's workspace, so the workspace holds as many block slots as can be
+// resident at once instead of one per block of the grid.
+__device__ __forceinline__ int acquire_slot(unsigned long long* bits, int nwords, int start) {
+  for (int w = start % nwords;; w = (w + 1 == nwords) ? 0 : w + 1) {
+    unsigned long long cur = __hip_atomic_load(bits + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (cur != ~0ull) {
+      const int b = __ffsll((long long)~cur) - 1;
+      const unsigned long long bit = 1ull << b;
+      const unsigned long long old = atomicOr(bits + w, bit);
+      if (!(old & bit)) return w * 64 + b;
+      cur = old | bit;
+    }
+  }
+}
+// The atomic's returned value is consumed, so the wave waits for the
+// release to complete before it moves on (and exits).
+__device__ __forceinline__ void release_slot(unsigned long long* bits, int slot) {
+  const unsigned long long old = atomicAnd(bits + (slot >> 6), ~(1ull << (slot & 63)));
+  asm volatile("" ::"v"(old));
+}
+
 hipError_t measure_fp64_peak(hipStream_t st, int cus, double* fp64_tflops);
 
 }  // namespace radm2

@@ -118,6 +118,7 @@ hipError_t RADM2_NAME(launch_persistent)(int block, int blocks, const LaunchArgs
   }
 }
 
+// resident blocks per CU, or minus the HIP error of the occupancy query
 int RADM2_NAME(persistent_blocks_per_cu)(int block) {
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
@@ -128,7 +129,7 @@ int RADM2_NAME(persistent_blocks_per_cu)(int block) {
 #endif
     default: break;
   }
-  return e == hipSuccess ? nb : -1;
+  return e == hipSuccess ? nb : -(int)e;
 }
 
 }  // namespace radm2

@@ -13,6 +13,21 @@ hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, h
   return hipGetLastError();
 }
 
+__global__ void k_slot_init(unsigned long long* bits, int nwords, int64_t nslots) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwords) return;
+  const int64_t lo = (int64_t)i * 64;
+  unsigned long long v = 0ull;
+  if (nslots <= lo) v = ~0ull;
+  else if (nslots < lo + 64) v = ~0ull << (nslots - lo);
+  bits[i] = v;
+}
+
+hipError_t launch_slot_init(unsigned long long* bits, int nwords, int64_t nslots, hipStream_t st) {
+  hipLaunchKernelGGL(k_slot_init, dim3((nwords + 63) / 64), dim3(64), 0, st, bits, nwords, nslots);
+  return hipGetLastError();
+}
+
 // Attainable FP64 peak for the roofline report (measurement only): eight
 // independent FMA chains per lane held in registers; the lane's sum is
 // stored so that no chain is dead.  b < 1 keeps the chains bounded.
@@ -67,7 +82,7 @@ done:
 const KernelVariant* kernel_variants() {
 #define RADM2_VARIANT_ROW_NAMED(P, W, R, F, S)                                              \
   {P, W, R, F, F ? WS_DOUBLES_FAMILY : WS_DOUBLES_ROS3, launch_grid##S, launch_persistent##S,   \
-   persistent_blocks_per_cu##S}
+   persistent_blocks_per_cu##S, grid_blocks_per_cu##S}
 #define RADM2_VARIANT_ROW(P, W, R) RADM2_VARIANT_ROW_NAMED(P, W, R, 0, _p##P##_w##W##_r##R)
   static const KernelVariant v[kNumVariants] = {
       RADM2_VARIANT_ROW(0, 1, 0), RADM2_VARIANT_ROW(0, 2, 0), RADM2_VARIANT_ROW(0, 4, 0),

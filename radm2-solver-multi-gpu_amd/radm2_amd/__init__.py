@@ -179,17 +179,17 @@ class CellBatch:
                  outputs=True, hfirst=False, jrate_end=None):
         import torch
         self.conc = _need(conc, "f64")
+        if conc.dim() != 2 or conc.shape[0] != NVAR:
+            raise ValueError("radm2: conc must be [59][ld], got %s" % (tuple(conc.shape),))
         self.ld = conc.shape[1]
         self.ncells = self.ld
-        self.temp = _need(temp, "f64")
-        self.cair = _need(cair, "f64")
-        self.h2o = _need(h2o, "f64")
-        self.jrate = _need(jrate, "f64")
-        assert conc.shape[0] == NVAR and jrate.shape == (NJ, self.ld)
-        self.rc_n2o5 = None if rc_n2o5 is None else _need(rc_n2o5, "f64")
-        self.h0 = None if h0 is None else _need(h0, "f64")
-        self.jrate_end = None if jrate_end is None else _need(jrate_end, "f64")
-        assert jrate_end is None or jrate_end.shape == (NJ, self.ld)
+        self.temp = self._cellwise(temp, "temp")
+        self.cair = self._cellwise(cair, "cair")
+        self.h2o = self._cellwise(h2o, "h2o")
+        self.jrate = self._species(jrate, NJ, "jrate")
+        self.rc_n2o5 = None if rc_n2o5 is None else self._cellwise(rc_n2o5, "rc_n2o5")
+        self.h0 = h0
+        self.jrate_end = None if jrate_end is None else self._species(jrate_end, NJ, "jrate_end")
         dev = conc.device
         if outputs:
             self.nstep = torch.zeros(self.ld, dtype=torch.int32, device=dev)
@@ -200,6 +200,32 @@ class CellBatch:
         else:
             self.nstep = self.nacc = self.nrej = self.status = self.hexit = None
         self.hfirst = torch.zeros(self.ld, dtype=torch.float64, device=dev) if hfirst else None
+
+    def _cellwise(self, t, name, kind="f64"):
+        """A per-cell array: device, contiguous, shape [ld]."""
+        _need(t, kind)
+        if tuple(t.shape) != (self.ld,):
+            raise ValueError("radm2: %s must have shape (%d,), got %s"
+                             % (name, self.ld, tuple(t.shape)))
+        if t.device != self.conc.device:
+            raise ValueError("radm2: %s is on %s, conc on %s" % (name, t.device, self.conc.device))
+        return t
+
+    def _species(self, t, rows, name):
+        _need(t, "f64")
+        if tuple(t.shape) != (rows, self.ld):
+            raise ValueError("radm2: %s must have shape (%d, %d), got %s"
+                             % (name, rows, self.ld, tuple(t.shape)))
+        return t
+
+    @property
+    def h0(self):
+        """Per-cell proposed initial step (f64 [ld]) or None."""
+        return self._h0
+
+    @h0.setter
+    def h0(self, t):
+        self._h0 = None if t is None else self._cellwise(t, "h0")
 
     def domain(self, dt, rtol, atol, opts=None, ncells=None):
         d = Domain()
@@ -213,6 +239,11 @@ class CellBatch:
         d.dt, d.rtol, d.atol = float(dt), float(rtol), float(atol)
         d.opts = ctypes.pointer(opts) if opts is not None else None
         d.jrate_end = _ptr(self.jrate_end)
+        if d.ncells < 0 or d.ncells > self.ld:
+            raise ValueError("radm2: ncells %d outside 0..%d" % (d.ncells, self.ld))
+        # the struct holds raw device pointers: keep the arrays (and the
+        # options it points to) alive as long as it is
+        d._owner = (self, opts)
         return d
 
 
@@ -299,7 +330,7 @@ class Solver:
         arr = (Domain * len(domains))(*domains)
         p = ctypes.c_void_p()
         _check(self.lib.radm2_plan_create(self.ctx, len(domains), arr, ctypes.byref(p)))
-        return Plan(self, p)
+        return Plan(self, p, domains)
 
     def sync(self):
         _check(self.lib.radm2_sync(self.ctx))
@@ -339,11 +370,14 @@ class Solver:
 
 
 class Plan:
-    """A pre-validated batch captured as a HIP graph (radm2_plan_*)."""
+    """A pre-validated batch captured as a HIP graph (radm2_plan_*).  It
+    keeps the domains it was built from, and through them their batches'
+    arrays, alive: the graph holds their device addresses."""
 
-    def __init__(self, solver, handle):
+    def __init__(self, solver, handle, domains=()):
         self.solver = solver
         self.handle = handle
+        self._domains = list(domains)
 
     def launch(self):
         self.solver._before()

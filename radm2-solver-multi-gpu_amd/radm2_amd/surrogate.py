@@ -127,13 +127,26 @@ def load(path, device="cuda"):
     return net, norm
 
 
+# per-cell status of a cell whose end state is the surrogate's prediction
+# (not a KPP code: those are 1 and -1..-8); its nstep/nacc/nrej and hexit
+# are 0
+STATUS_PREDICTED = 2
+
+
 def solve(solver, net, norm, batch, dt, rtol=1e-3, atol=1.0, thr=1e-3, audit=0.05, seed=0):
     """Surrogate pass over `batch` (in place): predicted cells take the
-    heads' mean, flagged and audited cells the FP64 solver's result.
-    Returns the flags (device int32, 1 = re-solved) and the re-solved count."""
+    heads' mean, flagged and audited cells the FP64 solver's result.  The
+    per-cell outputs: the solver's for re-solved cells; STATUS_PREDICTED,
+    zero counts and hexit 0 for predicted cells.  Returns the flags (device
+    int32, 1 = re-solved) and the re-solved count."""
     import torch
     n = batch.ncells
     dev = batch.conc.device
+    for name, v in (("nstep", 0), ("nacc", 0), ("nrej", 0), ("status", STATUS_PREDICTED),
+                    ("hexit", 0.0)):
+        dst = getattr(batch, name)
+        if dst is not None:
+            dst[:n].fill_(v)
     X, _ = mlp.features(batch, norm["mu"], norm["sd"])
     pad = (-n) % 16
     if pad:
@@ -167,11 +180,11 @@ def solve(solver, net, norm, batch, dt, rtol=1e-3, atol=1.0, thr=1e-3, audit=0.0
     st = mlp._st()
     mlp._ck(L.radm2_mlp_scatter(st, 8, 59, m, mlp._p(sb.conc), m, mlp._p(idx), batch.ld,
                                 mlp._p(batch.conc)))
-    for name in ("nstep", "nacc", "nrej", "status"):
+    for name, size in (("nstep", 4), ("nacc", 4), ("nrej", 4), ("status", 4), ("hexit", 8)):
         dst = getattr(batch, name)
         if dst is not None:
-            mlp._ck(L.radm2_mlp_scatter(st, 4, 1, m, mlp._p(getattr(sb, name)), m, mlp._p(idx),
-                                        batch.ld, mlp._p(dst)))
+            mlp._ck(L.radm2_mlp_scatter(st, size, 1, m, mlp._p(getattr(sb, name)), m,
+                                        mlp._p(idx), batch.ld, mlp._p(dst)))
     return flag, m
 
 

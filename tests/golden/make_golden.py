@@ -13,13 +13,17 @@ Blocks evaluated (reference line numbers):
   rates A[] and tendencies Vdot[]                   mgpu.cu:1103-1385
   Jacobian partials B[] and entries JVS[]           mgpu.cu:1685-2651
   unrolled forward/backward substitution            mgpu.cu:2873-3090
+  step-size control of acceptStep                   mgpu.cu:3238-3260
+    (with MAX/MIN/ONE as mgpu.cu:11, 191-192 define them), run on
+    scripted sequences of error sums, option sets and ELO values
 
 The decomposition kernel (mgpu.cu:2740-2779) indexes tables that the
 reference never fills (LU_CROW/LU_DIAG/LU_ICOL); its loop is restated here
 with those tables rebuilt from the solve text, so the LU fixture pins the
 unrolled decomposition against an independent loop.
 
-Output: ``radm2_golden.npz`` (plain arrays, no pickles) beside this script.
+Output: ``radm2_golden.npz`` and ``radm2_golden_control.npz`` (plain
+arrays, no pickles) beside this script.
 Run only where the reference is mounted:
     python make_golden.py /root/reference/mgpu.cu
 """
@@ -132,6 +136,135 @@ def kpp_decomp(JVS, crow, diag, icol):
     return 0
 
 
+def macro_fn(src, name):
+    """A function-like macro of the reference (``#define MAX(a,b) ...``) as a
+    Python function: its single ternary ``c ? x : y`` becomes ``x if c else y``."""
+    line = next(ln for ln in src if re.match(r"#define\s+%s\(" % name, ln))
+    m = re.match(r"#define\s+\w+\(([^)]*)\)\s*(.*)$", line)
+    args = [a.strip() for a in m.group(1).split(",")]
+    body = m.group(2).strip()
+    assert body[0] == "(" and body[-1] == ")", body
+    cond, rest = body[1:-1].split("?")
+    a, b = rest.split(":")
+    return eval("lambda %s: (%s) if (%s) else (%s)" % (", ".join(args), a, cond, b))
+
+
+def macro_const(src, name):
+    line = next(ln for ln in src if re.match(r"#define\s+%s\s" % name, ln))
+    return float(eval(line.split(None, 2)[2].replace("(double)", "")))
+
+
+def c_block_to_python(lines):
+    """Translate a small block of C statements (assignments, if / else with
+    or without braces, counted for loops) into Python source.  Lines inside
+    #ifdef ... #endif are dropped (the reference builds without those
+    flags)."""
+    out, ind, skip = [], 0, 0
+    for raw in lines:
+        ln = raw.strip()
+        if ln.startswith("#ifdef") or ln.startswith("#if "):
+            skip += 1
+            continue
+        if ln.startswith("#endif"):
+            skip -= 1
+            continue
+        if skip or not ln:
+            continue
+        ln = re.sub(r"MY_I\((\w+)\)", r"\1", ln)
+        ln = ln.replace("(double)", "").replace("||", " or ").replace("&&", " and ")
+        ln = re.sub(r"\bsqrt\(", "math.sqrt(", ln)
+        ln = re.sub(r"\bpow\(", "math.pow(", ln)
+        if ln == "}":
+            ind -= 1
+            continue
+        if re.match(r"\}\s*else\s*\{$", ln):
+            out.append("    " * (ind - 1) + "else:")
+            continue
+        m = re.match(r"for\s*\(\s*(\w+)\s*=\s*(\w+)\s*;\s*\w+\s*<\s*(\w+)\s*;\s*\w+\+\+\s*\)\s*\{$", ln)
+        if m:
+            out.append("    " * ind + "for %s in range(%s, %s):" % m.groups())
+            ind += 1
+            continue
+        m = re.match(r"if\s*(\(.*\))\s*\{$", ln)
+        if m:
+            out.append("    " * ind + "if %s:" % m.group(1))
+            ind += 1
+            continue
+        m = re.match(r"if\s*(\(.*?\))\s*([^()]+?=.*);$", ln)
+        if m:
+            out.append("    " * ind + "if %s:" % m.group(1))
+            out.append("    " * (ind + 1) + m.group(2).strip())
+            continue
+        assert ln.endswith(";"), ln
+        out.append("    " * ind + ln[:-1].strip())
+    assert ind == 0, ind
+    return "\n".join(out)
+
+
+# option sets (Hmin, FacMin, FacMax, FacRej, FacSafe) and ELO for the
+# controller fixtures: KPP's defaults at Ros3's order, other factors at
+# order 2, and a positive Hmin at order 4 (its "accept when H <= Hmin" arm)
+CONTROL_SETS = (
+    ((0.0, 0.2, 6.0, 0.1, 0.9), 3.0, 6.0),
+    ((0.0, 0.5, 3.0, 0.3, 0.8), 2.0, 60.0),
+    ((1.0e-3, 0.2, 6.0, 0.1, 0.9), 4.0, 0.05),
+)
+
+
+def control_sequences(rng):
+    """Scripted scaled errors Err per attempt: fixed patterns (accepts,
+    runs of one to five rejections, Err exactly 1, just above and below
+    1, 0, tiny and huge values) and random runs."""
+    fixed = [
+        [0.5, 0.5, 0.5],
+        [2.0, 0.5, 0.7],
+        [2.0, 3.0, 0.5, 0.4],
+        [2.0, 3.0, 4.0, 0.5, 0.5],
+        [2.0, 3.0, 4.0, 5.0, 6.0, 0.5, 2.0, 0.5],
+        [1.0, 1.0 + 2.0 ** -52, 1.0 - 2.0 ** -53, 0.999, 1.001],
+        [0.0, 1e-300, 1e-12, 1e6, 1e300, 0.5],
+        [50.0, 0.3, 9.0, 9.0, 9.0, 9.0, 9.0, 9.0, 9.0, 9.0, 9.0, 9.0, 9.0, 9.0, 0.01],
+    ]
+    rand = []
+    for _ in range(24):
+        n = int(rng.integers(10, 40))
+        e = 10 ** rng.uniform(-3.0, 1.5, n)
+        # runs of rejections: force a few stretches above 1
+        for a in rng.integers(0, n, 3):
+            e[a:a + int(rng.integers(1, 5))] = 10 ** rng.uniform(0.01, 1.0)
+        rand.append(list(e))
+    return fixed + rand
+
+
+def control_fixture(src, rng):
+    """acceptStep's control block (mgpu.cu:3238-3260) evaluated as written
+    on every (option set, sequence): per attempt the input error sum and
+    H, T, RejectLastH, RejectMoreH and the accepted flag after it."""
+    MAX, MIN = macro_fn(src, "MAX"), macro_fn(src, "MIN")
+    ONE = macro_const(src, "ONE")
+    code = c_block_to_python(src[3237:3260])
+    rows = []   # set, sequence, step, err_sum, H, T, rej_last, rej_more, accepted
+    seqs = control_sequences(rng)
+    for si, (fac, elo, h0) in enumerate(CONTROL_SETS):
+        hmin, facmin, facmax, facrej, facsafe = (np.float64(v) for v in fac)
+        for qi, errs in enumerate(seqs):
+            st = dict(H=[np.float64(h0)], T=[np.float64(0.0)], RejectLastH=[0], RejectMoreH=[0],
+                      vecMask2=[0], Y=[0.0] * 59, Ynew=[0.0] * 59)
+            for t, e in enumerate(errs):
+                err_sum = np.float64(59.0) * np.float64(e) * np.float64(e)
+                ns = dict(st, math=math, MAX=MAX, MIN=MIN, ONE=ONE, NVAR=59, p=0,
+                          Err=err_sum, Hmin=hmin, FacMin=facmin, FacMax=facmax, FacRej=facrej,
+                          FacSafe=facsafe, ros_ELO=np.float64(elo), Direction=1)
+                with np.errstate(divide="ignore", over="ignore"):
+                    exec(code, ns)
+                rows.append((si, qi, t, float(err_sum), float(ns["H"][0]), float(ns["T"][0]),
+                             ns["RejectLastH"][0], ns["RejectMoreH"][0],
+                             1 if ns["vecMask2"][0] == 0 else 0))
+    a = np.array(rows, dtype=np.float64)
+    sets = np.array([list(f) + [elo, h0] for f, elo, h0 in CONTROL_SETS])
+    return dict(CTRL_ROWS=a, CTRL_SETS=sets, CTRL_CODE=np.frombuffer(code.encode(), np.uint8))
+
+
 def main(path):
     src = open(path).read().split("\n")
     helpers = helper_functions(src)
@@ -194,6 +327,10 @@ def main(path):
                         RCONST=RCONST, VDOT=VDOT, JVS=JAC, LU=LU, SOLVE_B=SOLB,
                         SOLVE_X=SOLX, LU_CROW=np.array(crow), LU_DIAG=np.array(diag),
                         LU_ICOL=np.array(icol))
+    print("wrote", dst, os.path.getsize(dst), "bytes")
+
+    dst = os.path.join(HERE, "radm2_golden_control.npz")
+    np.savez_compressed(dst, **control_fixture(src, np.random.default_rng(20261204)))
     print("wrote", dst, os.path.getsize(dst), "bytes")
 
 

@@ -68,6 +68,14 @@ def lib():
         L.oracle_method_coeffs.argtypes = [ctypes.c_int] + [_D] * 7
         L.oracle_method_coeffs.restype = None
         L.oracle_photo_at.argtypes = [_D, _D, _D, ctypes.c_double, ctypes.c_double]
+        L.oracle_step_control.argtypes = [ctypes.c_double, ctypes.c_double,
+                                          ctypes.POINTER(OptsOut), _D, _D, _I, _I]
+        L.oracle_step_control.restype = ctypes.c_int
+        L.oracle_ros_attempt.argtypes = [ctypes.c_int, _D, _D] + [ctypes.c_double] * 5 + [_D, _D]
+        L.oracle_ros_attempt.restype = ctypes.c_double
+        L.oracle_ros_attempt_tab.argtypes = ([ctypes.c_int, _D, _D, _D, _D, _I, ctypes.c_double,
+                                              _D, _D] + [ctypes.c_double] * 5 + [_D, _D])
+        L.oracle_ros_attempt_tab.restype = ctypes.c_double
         L.oracle_photo_at.restype = None
         L.oracle_integrate.argtypes = [ctypes.c_int64, ctypes.c_int64, _D, _D, _D, _D, _D, _D, _D,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -138,6 +146,33 @@ def method_coeffs(method):
     lib().oracle_method_coeffs(method, *[_p(a) for a in arr])
     return dict(zip(("A", "C", "M", "E", "alpha", "gamma", "g_elo"), arr),
                 S=lib().oracle_method_stages(method))
+
+
+def step_control(err_sum, elo, opts, H, T, rej_last, rej_more):
+    """oracle_step_control on one attempt: (accepted, H, T, rej_last, rej_more)."""
+    h, t = ctypes.c_double(H), ctypes.c_double(T)
+    rl, rm = ctypes.c_int32(rej_last), ctypes.c_int32(rej_more)
+    acc = lib().oracle_step_control(err_sum, elo, ctypes.byref(opts), ctypes.byref(h),
+                                    ctypes.byref(t), ctypes.byref(rl), ctypes.byref(rm))
+    return acc, h.value, t.value, rl.value, rm.value
+
+
+def ros_attempt(method, y, k, h2o, m, H, rtol=1e-3, atol=1.0, table=None):
+    """One autonomous attempt (oracle_ros_attempt, or oracle_ros_attempt_tab
+    with `table` = method_coeffs()-style dict plus "newf"): (err_sum, Ynew,
+    sum_j E_j K_j)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    k = np.ascontiguousarray(k, dtype=np.float64)
+    yn, ye = np.zeros(59), np.zeros(59)
+    if table is None:
+        e = lib().oracle_ros_attempt(method, _p(y), _p(k), h2o, m, H, rtol, atol, _p(yn), _p(ye))
+    else:
+        c = {q: np.ascontiguousarray(table[q], dtype=np.float64) for q in "ACME"}
+        nf = np.ascontiguousarray(table["newf"], dtype=np.int32)
+        e = lib().oracle_ros_attempt_tab(int(table["S"]), _p(c["A"]), _p(c["C"]), _p(c["M"]),
+                                         _p(c["E"]), _p(nf), float(table["g_elo"][0]), _p(y),
+                                         _p(k), h2o, m, H, rtol, atol, _p(yn), _p(ye))
+    return e, yn, ye
 
 
 def integrate(cells, dt, rtol, atol, opts=None, h0=None, nthreads=0, n=None, use_rc=True,

@@ -36,18 +36,21 @@ def _compare(cg, og, co, oo, label):
 
 
 @pytest.mark.parametrize("schedule", [0, 1])
-def test_parity_config0_subset(solver, schedule):
+def test_parity_config0_full(solver, schedule):
+    # the whole 4,096-cell workload (16x16x16, seed 0, zenith U(0,80)), both
+    # schedules: every cell within 1e-6 of the oracle with the same step,
+    # accept and reject counts (device and host exp/pow may differ by an
+    # ulp in a rate constant; on these seeded inputs no decision flips)
     from radm2_amd import synth
-    cells = synth.make_cells(1024, 0, 80.0, 0.0)
+    cells = synth.make_cells(**synth.CONFIGS["c0_4k"])
+    assert cells["temp"].shape == (4096,)
     co, oo, code = O.integrate(cells, 600.0, 1e-3, 1.0, nthreads=0)
     assert code == 0
     cg, og = _run_gpu(solver, cells, 600.0, 1e-3, 1.0, schedule)
     assert (og["status"] == 1).all()
     same, ok, rel = _compare(cg, og, co, oo, "sched %d" % schedule)
-    # identical rate constants would give identical bits; libm vs device
-    # exp/pow differ by an ulp, which can flip a rare accept/reject decision
-    assert ok.mean() >= 0.99
-    assert same.mean() >= 0.99
+    assert ok.all(), np.flatnonzero(~ok)
+    assert same.all(), np.flatnonzero(~same)
 
 
 def test_parity_tight_tolerance_reference_run(solver):
@@ -80,7 +83,7 @@ def test_other_kpp_methods_match_oracle(solver, method):
             cg, og = _run_gpu(solver, cells, 600.0, 1e-3, 1.0, sched, opts=radm2_amd.Opts(method=method))
             assert (og["status"] == 1).all()
             same, ok, rel = _compare(cg, og, co, oo, "method %d schedule %d" % (method, sched))
-            assert ok.mean() >= 0.99 and same.mean() >= 0.99
+            assert ok.all() and same.all()
         for pl, w in ((radm2_amd.PLACE_PRIVATE, 4), (radm2_amd.PLACE_WORKSPACE, 1),
                       (radm2_amd.PLACE_LDS160, 1)):
             solver.set_kernel(pl, w)
@@ -122,7 +125,7 @@ def test_mixed_method_domains_on_the_family_kernel(solver):
     co, oo, code = O.integrate(cb, 300.0, 1e-3, 1.0, opts=O.OptsIn(method=4), nthreads=0)
     same, ok, rel = _compare(b1.conc.cpu().numpy(), dict(nstep=out(b1)[1], nacc=out(b1)[2],
                                                          nrej=out(b1)[3]), co, oo, "mixed")
-    assert ok.mean() >= 0.99 and same.mean() >= 0.99
+    assert ok.all() and same.all()
 
 
 def _forced_cells(n, seed):
@@ -162,7 +165,7 @@ def test_nonautonomous_forcing_matches_oracle(solver, method):
         assert (b.status.cpu().numpy() == 1).all()
         same, ok, rel = _compare(b.conc.cpu().numpy(), og, co, oo,
                                  "forced method %d schedule %d" % (method, sched))
-        assert ok.mean() >= 0.99 and same.mean() >= 0.99
+        assert ok.all() and same.all()
     solver.set_schedule(radm2_amd.SCHED_PERSISTENT)
 
 

@@ -73,7 +73,7 @@ def test_all_variants_and_schedules_bit_identical(solver, cells):
             ref = out
             co, oo, code = O.integrate(sub, 600.0, 1e-3, 1.0)
             assert code == 0 and (out["status"] == 1).all()
-            assert _close(out["conc"], co).mean() >= 0.99
+            assert _close(out["conc"], co).all()
         for k in ref:
             np.testing.assert_array_equal(out[k], ref[k], err_msg=str((pl, w, sched, block, k)))
     solver.set_schedule(1, 0, 0)
@@ -106,6 +106,10 @@ def test_lds_placements_reject_wide_blocks(solver, cells):
         with pytest.raises(radm2_amd.Radm2Error) as e:
             solver.integrate(b, 600.0, 1e-3, 1.0)
         assert e.value.code == radm2_amd.E_ARG
+        # a plan checks the same launch shape before it captures anything
+        with pytest.raises(radm2_amd.Radm2Error) as e:
+            solver.plan([b.domain(600.0, 1e-3, 1.0)])
+        assert e.value.code == radm2_amd.E_ARG
     solver.set_schedule(1, 0, 0)
     solver.set_kernel(-1, 0)
 
@@ -126,9 +130,9 @@ def test_batched_domains_match_oracle_per_domain(solver):
         co, oo, code = O.integrate(c, dt, rtol, atol)
         out = _out(b)
         assert (out["status"] == 1).all()
-        assert _close(out["conc"], co).mean() >= 0.99, (n, dt)
+        assert _close(out["conc"], co).all(), (n, dt)
         same = (out["nstep"] == oo["nstep"]) & (out["nrej"] == oo["nrej"])
-        assert same.mean() >= 0.99
+        assert same.all()
 
 
 def test_plan_graph_replays_match_eager(solver, cells):
@@ -201,8 +205,8 @@ def test_options_and_h0_match_oracle(solver, cells):
     co, oo, code = O.integrate(sub, 600.0, 1e-3, 1.0, opts=O.OptsIn(**kw))
     out = _out(b)
     assert code == 0 and (out["status"] == 1).all()
-    assert _close(out["conc"], co).mean() >= 0.99
-    assert (out["nstep"] == oo["nstep"]).mean() >= 0.99
+    assert _close(out["conc"], co).all()
+    assert (out["nstep"] == oo["nstep"]).all()
     # per-cell Hstart: min(h0, dt, Hmax), DeltaMin below 10 roundoff
     h0 = np.geomspace(1e-16, 1e3, 300)
     b = _batch(sub, h0=h0)
@@ -210,8 +214,8 @@ def test_options_and_h0_match_oracle(solver, cells):
     solver.sync()
     co, oo, code = O.integrate(sub, 600.0, 1e-3, 1.0, h0=h0)
     out = _out(b)
-    assert _close(out["conc"], co).mean() >= 0.99
-    assert (out["nstep"] == oo["nstep"]).mean() >= 0.99
+    assert _close(out["conc"], co).all()
+    assert (out["nstep"] == oo["nstep"]).all()
 
 
 def test_failure_statuses_isolated(solver, cells):
@@ -230,7 +234,7 @@ def test_failure_statuses_isolated(solver, cells):
     np.testing.assert_array_equal(out["status"], oo["status"])
     ok = out["status"] == 1
     assert ok.sum() > 10
-    assert _close(out["conc"][:, ok], co[:, ok]).mean() >= 0.99
+    assert _close(out["conc"][:, ok], co[:, ok]).all()
 
 
 def test_call_level_errors(solver, cells):
@@ -311,7 +315,7 @@ def test_batched_64_uneven_domains_plan_matches_oracle(solver):
         el = time.perf_counter() - t
     print("64 domains, %d cells: %.3f s, %.0f cell-integrations/s" % (total, el, total / el))
     nsub = 32
-    worst, good = 1.0, 0
+    good = 0
     for b, (c, dt, rtol, atol) in zip(batches, doms):
         st = b.status.cpu().numpy()
         assert (st == 1).all(), (dt, rtol)
@@ -320,9 +324,9 @@ def test_batched_64_uneven_domains_plan_matches_oracle(solver):
         co, oo, code = O.integrate(sub, dt, rtol, atol)
         assert code == 0
         ok = _close(b.conc[:, :nsub].cpu().numpy(), co)
-        worst = min(worst, ok.mean())
+        ok &= b.nstep[:nsub].cpu().numpy() == oo["nstep"]
         good += int(ok.sum())
-    assert good >= 0.98 * nsub * len(doms) and worst >= 0.9, (good, worst)
+    assert good == nsub * len(doms), good
     plan.close()
 
 

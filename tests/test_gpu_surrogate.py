@@ -91,6 +91,21 @@ def test_resolved_cells_are_the_solvers_and_audit_fraction(solver, trained):
     f = r["surrogate"]["flag"].astype(bool)
     assert abs(f.mean() - 0.05) < 4 * np.sqrt(0.05 * 0.95 / 16384)
     np.testing.assert_array_equal(r["surrogate"]["conc"][:, f], r["plain"]["conc"][:, f])
+    # per-cell outputs: the solver's for re-solved cells, STATUS_PREDICTED
+    # and zero counts for predicted ones, whatever the batch held before
+    from radm2_amd import learned_h0
+    b = learned_h0._cells_to_batch(held)
+    b.nstep.fill_(77)
+    b.status.fill_(9)
+    b.hexit.fill_(-1.0)
+    flag, m = surrogate.solve(solver, net, norm, b, 600.0, thr=1e30, audit=0.05)
+    solver.sync()
+    f = flag.cpu().numpy().astype(bool)
+    st, ns, hx = b.status.cpu().numpy(), b.nstep.cpu().numpy(), b.hexit.cpu().numpy()
+    assert m == f.sum() > 0
+    assert (st[~f] == surrogate.STATUS_PREDICTED).all() and (ns[~f] == 0).all()
+    assert (hx[~f] == 0.0).all()
+    assert (st[f] == 1).all() and (ns[f] > 0).all() and (hx[f] > 0.0).all()
 
 
 def test_surrogate_error_and_fallback_report(solver, trained):
