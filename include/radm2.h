@@ -148,6 +148,33 @@ int radm2_integrate(radm2_ctx *ctx, int64_t ncells, int64_t ld, double *conc,
                     double atol, const radm2_opts *opts, const double *h0, int32_t *nstep,
                     int32_t *nacc, int32_t *nrej, int32_t *status, double *hexit);
 
+/*
+ * radm2_integrate without any device allocation: the persistent kernel's
+ * workspace, its work counter and the domain descriptor live in a
+ * file-scope device array of the library (128 MiB per device, held from the
+ * first radm2_create on that device until the process ends), and the call
+ * issues kernels only (no hipMalloc/hipFree, no copy from host memory, no
+ * host synchronisation).  No counterpart in the reference: it is for
+ * callers that cannot allocate -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.
+ * Same arguments, semantics and results as radm2_integrate, always on the
+ * persistent schedule (the context's kernel variant; the context's block
+ * size when its schedule is persistent, else 64).  The array bounds the
+ * resident waves, not the cell count: at most 236 waves of the Ros3
+ * kernels (194 of the family kernel) run at once, about half of what
+ * radm2_integrate uses on MI355X, so large batches take longer.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other contexts, other host threads) race and corrupt each other.
+ * Returns RADM2_E_UNSUPPORTED if one block of the configured variant does
+ * not fit the array.
+ */
+int radm2_integrate_noalloc(radm2_ctx *ctx, int64_t ncells, int64_t ld, double *conc,
+                            const double *temp, const double *cair, const double *h2o,
+                            const double *jrate, const double *rc_n2o5, double dt, double rtol,
+                            double atol, const radm2_opts *opts, const double *h0,
+                            int32_t *nstep, int32_t *nacc, int32_t *nrej, int32_t *status,
+                            double *hexit);
+
 /* Several independent domains (own arrays, dt, tolerances) in one launch.
  * No reference counterpart beyond its single hard-coded 39x19x39 domain
  * (mgpu.cu:420, 651-653); each domain follows radm2_integrate's semantics. */

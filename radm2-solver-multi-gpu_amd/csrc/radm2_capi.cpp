@@ -42,6 +42,7 @@ struct radm2_ctx {
   bool stats_on = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_staged = nullptr;
   bool timed = false;
+  void* scratch = nullptr;  // this device's array of radm2_integrate_noalloc
 };
 
 struct radm2_plan {
@@ -224,8 +225,14 @@ struct Shape {
   int64_t blocks = 0, waves = 0, slots = 0;
 };
 
-int launch_shape(const radm2_ctx* c, int64_t total, bool family, Shape* sh) {
-  sh->block = pick_block(c);
+// sched: the schedule (the context's when negative); max_waves > 0 caps the
+// persistent kernel's resident waves (a workspace of that many wave slots).
+int launch_shape(const radm2_ctx* c, int64_t total, bool family, Shape* sh, int sched = -1,
+                 int64_t max_waves = 0) {
+  if (sched < 0) sched = c->schedule;
+  // a schedule other than the context's runs at that schedule's default block
+  sh->block = sched == c->schedule ? pick_block(c)
+              : (sched == RADM2_SCHED_GRID ? kDefaultGridBlock : kDefaultPersistentBlock);
   sh->v = pick_variant(c, family);
   if (!sh->v) return family ? RADM2_E_UNSUPPORTED : RADM2_E_ARG;
   const radm2::KernelVariant* v = sh->v;
@@ -234,7 +241,7 @@ int launch_shape(const radm2_ctx* c, int64_t total, bool family, Shape* sh) {
   const int64_t wpb = block / radm2::WAVE;
   const int64_t need = (total + block - 1) / block;
   int per_cu = 0;
-  if (c->schedule == RADM2_SCHED_GRID) {
+  if (sched == RADM2_SCHED_GRID) {
     sh->blocks = need;
     if (v->placement >= radm2::PLACE_WS) {
       if (int rc = resident_per_cu(v->grid_blocks_per_cu, block, &per_cu)) return rc;
@@ -245,6 +252,10 @@ int launch_shape(const radm2_ctx* c, int64_t total, bool family, Shape* sh) {
   } else {
     if (int rc = resident_per_cu(v->blocks_per_cu, block, &per_cu)) return rc;
     sh->blocks = std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->cu_count, need));
+    if (max_waves > 0 && v->placement >= radm2::PLACE_WS) {
+      sh->blocks = std::min<int64_t>(sh->blocks, max_waves / wpb);
+      if (sh->blocks < 1) return RADM2_E_UNSUPPORTED;
+    }
     sh->waves = sh->blocks * wpb;
   }
   return 0;
@@ -254,9 +265,12 @@ int launch_shape(const radm2_ctx* c, int64_t total, bool family, Shape* sh) {
 // workspace to use (grown here when the explicit placement needs more);
 // grow = false for graph capture, where the plan sized it beforehand.
 // slots: the grid schedule's block-slot bitmap (kSlotWords words).
+// sched, max_waves: as launch_shape.
 int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
             unsigned long long* counter, unsigned long long* slots, hipStream_t st, double** ws,
-            size_t* ws_bytes, bool family, bool grow = true) {
+            size_t* ws_bytes, bool family, bool grow = true, int sched = -1,
+            int64_t max_waves = 0) {
+  if (sched < 0) sched = c->schedule;
   LaunchArgs a;
   a.doms = d_doms;
   a.ndom = ndom;
@@ -267,7 +281,7 @@ int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
   a.slot_bits = slots;
   a.slot_words = 0;
   Shape sh;
-  if (int rc = launch_shape(c, total, family, &sh)) return rc;
+  if (int rc = launch_shape(c, total, family, &sh, sched, max_waves)) return rc;
   const radm2::KernelVariant* v = sh.v;
   if (v->placement >= radm2::PLACE_WS) {
     if (grow) {
@@ -281,7 +295,7 @@ int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
     a.ws = nullptr;
     a.ws_waves = 0;
   }
-  if (c->schedule == RADM2_SCHED_GRID) {
+  if (sched == RADM2_SCHED_GRID) {
     if (sh.slots > 0) {
       a.slot_words = (int32_t)((sh.slots + 63) / 64);
       HIP_TRY(radm2::launch_slot_init(slots, a.slot_words, sh.slots, st));
@@ -334,6 +348,8 @@ int radm2_create(radm2_ctx** ctx, int device, void* stream) {
   if (e != hipSuccess) return fail(e);
   e = hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) return fail(e);
+  // this device's copy of the allocation-free entry point's array
+  if ((e = radm2::noalloc_scratch(device, &c->scratch)) != hipSuccess) return fail(e);
   if (stream) {
     c->stream = (hipStream_t)stream;
   } else {
@@ -456,6 +472,61 @@ int radm2_integrate(radm2_ctx* c, int64_t ncells, int64_t ld, double* conc, cons
   d.atol = atol;
   d.opts = opts;
   return radm2_integrate_batched(c, 1, &d);
+}
+
+int radm2_integrate_noalloc(radm2_ctx* c, int64_t ncells, int64_t ld, double* conc,
+                            const double* temp, const double* cair, const double* h2o,
+                            const double* jrate, const double* rc_n2o5, double dt, double rtol,
+                            double atol, const radm2_opts* opts, const double* h0,
+                            int32_t* nstep, int32_t* nacc, int32_t* nrej, int32_t* status,
+                            double* hexit) {
+  if (!c || !c->scratch) return RADM2_E_ARG;
+  radm2_domain d;
+  std::memset(&d, 0, sizeof d);
+  d.ncells = ncells;
+  d.ld = ld;
+  d.conc = conc;
+  d.temp = temp;
+  d.cair = cair;
+  d.h2o = h2o;
+  d.jrate = jrate;
+  d.rc_n2o5 = rc_n2o5;
+  d.h0 = h0;
+  d.nstep = nstep;
+  d.nacc = nacc;
+  d.nrej = nrej;
+  d.status = status;
+  d.hexit = hexit;
+  d.dt = dt;
+  d.rtol = rtol;
+  d.atol = atol;
+  d.opts = opts;
+  std::vector<DomainDesc> tab;
+  int64_t total = 0;
+  bool family = false;
+  if (int rc = build_table(c, &d, 1, tab, &total, &family)) return rc;
+  c->timed = false;
+  if (total == 0) return RADM2_OK;
+  HIP_TRY(hipSetDevice(c->device));
+  // the array: counter, domain descriptor, workspace (no allocation, no
+  // copy from host memory: the descriptor goes in as a kernel argument)
+  unsigned char* base = static_cast<unsigned char*>(c->scratch);
+  auto* counter = reinterpret_cast<unsigned long long*>(base);
+  auto* d_dom = reinterpret_cast<DomainDesc*>(base + radm2::kNoallocDomOffset);
+  double* ws = reinterpret_cast<double*>(base + radm2::kNoallocWsOffset);
+  size_t ws_bytes = radm2::kNoallocScratchBytes - radm2::kNoallocWsOffset;
+  const radm2::KernelVariant* v = pick_variant(c, family);
+  if (!v) return family ? RADM2_E_UNSUPPORTED : RADM2_E_ARG;
+  const int64_t max_waves =
+      (int64_t)(ws_bytes / ((size_t)v->ws_doubles * radm2::WAVE * sizeof(double)));
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  HIP_TRY(radm2::launch_put_domain(tab[0], d_dom, c->stream));
+  if (int rc = enqueue(c, d_dom, 1, total, counter, nullptr, c->stream, &ws, &ws_bytes, family,
+                       false, RADM2_SCHED_PERSISTENT, max_waves))
+    return rc;
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  return RADM2_OK;
 }
 
 int radm2_plan_create(radm2_ctx* c, int ndom, const radm2_domain* doms, radm2_plan** out) {

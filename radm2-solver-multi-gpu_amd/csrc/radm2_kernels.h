@@ -152,4 +152,17 @@ __device__ __forceinline__ void release_slot(unsigned long long* bits, int slot)
 
 hipError_t measure_fp64_peak(hipStream_t st, int cus, double* fp64_tflops);
 
+// The allocation-free entry point's device array (radm2_noalloc.hip): the
+// work counter at byte 0, the domain descriptor at kNoallocDomOffset, the
+// workspace from kNoallocWsOffset to the end.
+constexpr size_t kNoallocScratchBytes = size_t(128) << 20;
+constexpr size_t kNoallocDomOffset = 256;
+constexpr size_t kNoallocWsOffset = 4096;
+static_assert(kNoallocDomOffset + sizeof(DomainDesc) <= kNoallocWsOffset, "scratch header");
+// This device's address of the array (looked up once per device, with the
+// device current).
+hipError_t noalloc_scratch(int device, void** ptr);
+// Writes `d` to device memory `dst` with a kernel (no copy from host memory).
+hipError_t launch_put_domain(const DomainDesc& d, DomainDesc* dst, hipStream_t st);
+
 }  // namespace radm2

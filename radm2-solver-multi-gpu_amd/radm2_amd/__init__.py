@@ -90,6 +90,9 @@ _EXPORTS = [
     ("radm2_integrate", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_void_p] * 6
      + [ctypes.c_double] * 3 + [ctypes.POINTER(Opts)] + [ctypes.c_void_p] * 6),
+    ("radm2_integrate_noalloc", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64] + [ctypes.c_void_p] * 6
+     + [ctypes.c_double] * 3 + [ctypes.POINTER(Opts)] + [ctypes.c_void_p] * 6),
     ("radm2_integrate_batched", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Domain)]),
     ("radm2_plan_create", ctypes.c_int,
      [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Domain), ctypes.POINTER(ctypes.c_void_p)]),
@@ -311,6 +314,20 @@ class Solver:
         integrate(), which goes through radm2_integrate_batched)."""
         self._before()
         code = self.lib.radm2_integrate(
+            self.ctx, batch.ncells, batch.ld, _ptr(batch.conc), _ptr(batch.temp),
+            _ptr(batch.cair), _ptr(batch.h2o), _ptr(batch.jrate), _ptr(batch.rc_n2o5),
+            float(dt), float(rtol), float(atol), ctypes.pointer(opts) if opts is not None else None,
+            _ptr(batch.h0), _ptr(batch.nstep), _ptr(batch.nacc), _ptr(batch.nrej),
+            _ptr(batch.status), _ptr(batch.hexit))
+        self._after()
+        _check(code)
+
+    def integrate_noalloc(self, batch, dt, rtol=1e-3, atol=1.0, opts=None):
+        """radm2_integrate_noalloc: radm2_integrate's results without any
+        device allocation (the library's per-device array holds the
+        workspace; overlapping calls on one GPU race)."""
+        self._before()
+        code = self.lib.radm2_integrate_noalloc(
             self.ctx, batch.ncells, batch.ld, _ptr(batch.conc), _ptr(batch.temp),
             _ptr(batch.cair), _ptr(batch.h2o), _ptr(batch.jrate), _ptr(batch.rc_n2o5),
             float(dt), float(rtol), float(atol), ctypes.pointer(opts) if opts is not None else None,
