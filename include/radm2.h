@@ -131,6 +131,12 @@ int radm2_set_schedule(radm2_ctx *ctx, int schedule, int block, int refill_min);
 #define RADM2_PLACE_LDS160 2
 #define RADM2_PLACE_LDS312 3
 #define RADM2_PLACE_LDS160_REG128 4
+/* A cell spread over 2 (SPLIT2) or 4 (SPLIT4) lanes that all run its
+ * computation: the workspace plus, per cell, 160 / 320 doubles in LDS and
+ * 256 / 512 in registers split over the cell's lanes (persistent schedule,
+ * 64-lane blocks; the grid schedule gives RADM2_E_UNSUPPORTED). */
+#define RADM2_PLACE_SPLIT2 5
+#define RADM2_PLACE_SPLIT4 6
 int radm2_set_kernel(radm2_ctx *ctx, int placement, int waves_per_simd);
 
 /*

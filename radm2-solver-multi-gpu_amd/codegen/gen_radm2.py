@@ -420,11 +420,11 @@ def gen_solve_pf(mech, G):
     ops = []          # (lu entry, statement using lu_<entry>)
     for i in range(59):
         for kk in range(crow[i], diag[i]):
-            ops.append((kk, "x%d = fma(-lu%d, x%d, x%d);" % (i, kk, icol[kk], i)))
+            ops.append((kk, "x%d = fma(-V(lu%d), x%d, x%d);" % (i, kk, icol[kk], i)))
     for i in range(58, -1, -1):
         for kk in range(diag[i] + 1, crow[i + 1]):
-            ops.append((kk, "x%d = fma(-lu%d, x%d, x%d);" % (i, kk, icol[kk], i)))
-        ops.append((diag[i], "x%d = x%d * lu%d;" % (i, i, diag[i])))
+            ops.append((kk, "x%d = fma(-V(lu%d), x%d, x%d);" % (i, kk, icol[kk], i)))
+        ops.append((diag[i], "x%d = x%d * V(lu%d);" % (i, i, diag[i])))
     assert sorted(e for e, _ in ops) == list(range(lu["nnz"]))
     groups = [ops[a:a + G] for a in range(0, len(ops), G)]
     fence = "  __builtin_amdgcn_sched_barrier(0);"
@@ -435,7 +435,7 @@ def gen_solve_pf(mech, G):
         out.append("  double x%d = RD(x, %d);" % (i, i))
 
     def loads(grp):
-        return ["  const double lu%d = RD(g, %d);" % (e, e) for e, _ in grp]
+        return ["  const auto lu%d = RDP<%d>(g);" % (e, e) for e, _ in grp]
     out += loads(groups[0])
     for m, grp in enumerate(groups):
         if m + 1 < len(groups):
@@ -449,12 +449,32 @@ def gen_solve_pf(mech, G):
     return "\n".join(out)
 
 
+def deferred_src(src):
+    """RD(a, i) -> RDP<i>(a): a read issued with its batch, except for an
+    entry in a register tier split over a cell's lanes, whose cross-lane
+    move RDP defers to the use (V(name)), so that it holds no register
+    between the batch and the use."""
+    import re
+    mo = re.fullmatch(r"RD\((\w+), (\d+)\)", src)
+    assert mo, src
+    return "RDP<%s>(%s)" % (mo.group(2), mo.group(1))
+
+
+def wrap_uses(stmt, names):
+    """Each use of a scheduled load's name in `stmt` as V(name)."""
+    import re
+    for nm in sorted(set(names), key=len, reverse=True):
+        stmt = re.sub(r"\b%s\b" % re.escape(nm), "V(%s)" % nm, stmt)
+    return stmt
+
+
 def schedule_loads(ops, loads, G):
     """Body lines for `ops` (statement, [load indices], stored entry or None)
     with the `loads` (name, source, producing entry or None) issued in
     batches of G behind scheduling fences, so that between G and 2G loads
     run ahead of their first use; a load waits for the op storing its
     entry.  Returns (lines, number of empty batches)."""
+    ops = [(wrap_uses(st, [loads[u][0] for u in used]), used, sto) for st, used, sto in ops]
     store_op = {st: i for i, (_, _, st) in enumerate(ops) if st is not None}
     fence = "  __builtin_amdgcn_sched_barrier(0);"
     out = []
@@ -468,7 +488,7 @@ def schedule_loads(ops, loads, G):
                 name, src, dep = loads[issued]
                 if dep is not None and store_op[dep] >= i:
                     break            # its producer is not emitted yet
-                batch.append("  const double %s = %s;" % (name, src))
+                batch.append("  const auto %s = %s;" % (name, deferred_src(src)))
                 issued += 1
             if not batch:
                 assert issued > nm
@@ -660,15 +680,15 @@ def hot_ranking(mech):
             for jj in range(diag[j] + 1, crow[j + 1]):
                 acc[(2, jj)] += 1
     text = gen_jac_decomp_pf(mech, 16)[0]
-    for mo in re.finditer(r"RD\((k|v), (\d+)\)", text):
-        acc[(0 if mo.group(1) == "k" else 1, int(mo.group(2)))] += 1
-    for mo in re.finditer(r"RD\((k|v), (\d+)\)", gen_fun_pf(mech, 16)):
-        if mo.group(1) == "k":
+    for mo in re.finditer(r"RDP<(\d+)>\((k|v)\)", text):
+        acc[(0 if mo.group(2) == "k" else 1, int(mo.group(1)))] += 1
+    for mo in re.finditer(r"RDP<(\d+)>\((k|v)\)", gen_fun_pf(mech, 16)):
+        if mo.group(2) == "k":
             # two tendencies per attempt; weighted 3, the measured best
             # (2: 770.7 ms, 3: 758.9 ms, 4: 773.2 ms at 1M cells)
-            acc[(0, int(mo.group(2)))] += 3
+            acc[(0, int(mo.group(1)))] += 3
         else:
-            acc[(1, int(mo.group(2)))] += 1
+            acc[(1, int(mo.group(1)))] += 1
     for i in range(59):
         acc[(1, i)] += 3
     keys = [(0, r) for r in range(mech["nreact"])] + [(1, i) for i in range(59)] + \
@@ -676,34 +696,58 @@ def hot_ranking(mech):
     return sorted(keys, key=lambda key: -acc[key]), acc
 
 
-# (LDS slots, register entries) per lane of each hot-set configuration
-HOT_BUDGETS = ((160, 0), (312, 0), (160, 128))
+# Hot-set configurations (S, R, P, L): S LDS slots per cell, R registers per
+# lane holding the same entries on every lane of a cell, P registers per lane
+# of a cell split over its L lanes (register r of lane q of the cell holds
+# split entry r*L + q, so a cell holds P*L entries there); L = 1: one cell
+# per lane
+HOT_BUDGETS = ((160, 0, 0, 1), (312, 0, 0, 1), (160, 128, 0, 1), (160, 0, 128, 2),
+               (320, 0, 128, 4))
+# code of a split-tier entry: SPLIT_BASE - (its index in the split tier)
+SPLIT_BASE = -(1 << 14)
+
+
+def hot_name(S, R, P, L):
+    return "HOT%d_%d_%dx%d" % (S, R, P, L)
+
+
+def hot_codes(mech, S, R, P, L):
+    """Per array (rate constants, state, LU), the code of each entry under
+    configuration (S, R, P, L): s >= 0 for LDS slot s (the first S of the
+    ranking), -2 - r for replicated register r (the next R), SPLIT_BASE - j
+    for split entry j (the next P*L; lane j % L of the cell, register
+    j // L), -1 for the HBM workspace."""
+    ranked, _ = hot_ranking(mech)
+    sizes = (mech["nreact"], 59, mech["lu"]["nnz"])
+    code = [[-1] * n for n in sizes]
+    for s_, (a, i) in enumerate(ranked[:S]):
+        code[a][i] = s_
+    for r, (a, i) in enumerate(ranked[S:S + R]):
+        code[a][i] = -2 - r
+    for j, (a, i) in enumerate(ranked[S + R:S + R + P * L]):
+        code[a][i] = SPLIT_BASE - j
+    return code
 
 
 def gen_hot_tables(mech):
-    """Per hot-set configuration (S LDS slots, R registers per lane): the
-    code of each rate constant, state and LU entry -- s >= 0 for LDS slot s
-    (the first S of the ranking), -2 - r for register r (the next R), -1
-    for the HBM workspace."""
-    ranked, _ = hot_ranking(mech)
+    """The code of every rate constant, state and LU entry per hot-set
+    configuration (hot_codes), and hot_code<S, R, P, L>(array, index)."""
     sizes = (mech["nreact"], 59, mech["lu"]["nnz"])
-    out = ["// hot set per (LDS slots, registers): code per entry (s >= 0 LDS slot s,"
-           " -2 - r register r, -1 HBM)"]
-    for S, R in HOT_BUDGETS:
-        code = [[-1] * n for n in sizes]
-        for s_, (a, i) in enumerate(ranked[:S]):
-            code[a][i] = s_
-        for r, (a, i) in enumerate(ranked[S:S + R]):
-            code[a][i] = -2 - r
+    out = ["// hot set per (LDS slots S, replicated registers R, split registers P, lanes per"
+           " cell L): code per entry (s >= 0 LDS slot s, -2 - r replicated register r,"
+           " %d - j split entry j, -1 HBM)" % SPLIT_BASE]
+    out.append("constexpr int HOT_SPLIT_BASE = %d;" % SPLIT_BASE)
+    for S, R, P, L in HOT_BUDGETS:
+        code = hot_codes(mech, S, R, P, L)
         for a, name in enumerate(("K", "Y", "G")):
-            out.append("__device__ constexpr int16_t HOT%d_%d_%s[%d] = {%s};"
-                       % (S, R, name, sizes[a], ",".join(map(str, code[a]))))
-    out.append("template <int S, int R>")
+            out.append("__device__ constexpr int16_t %s_%s[%d] = {%s};"
+                       % (hot_name(S, R, P, L), name, sizes[a], ",".join(map(str, code[a]))))
+    out.append("template <int S, int R, int P, int L>")
     out.append("__device__ __forceinline__ constexpr int hot_code(int arr, int i) {")
-    for S, R in HOT_BUDGETS:
-        out.append("  if constexpr (S == %d && R == %d)" % (S, R))
-        out.append("    return arr == 0 ? HOT%d_%d_K[i] : arr == 1 ? HOT%d_%d_Y[i] : HOT%d_%d_G[i];"
-                   % (S, R, S, R, S, R))
+    for S, R, P, L in HOT_BUDGETS:
+        n = hot_name(S, R, P, L)
+        out.append("  if constexpr (S == %d && R == %d && P == %d && L == %d)" % (S, R, P, L))
+        out.append("    return arr == 0 ? %s_K[i] : arr == 1 ? %s_Y[i] : %s_G[i];" % (n, n, n))
     out.append("  return -1;")
     out.append("}")
     return "\n".join(out)

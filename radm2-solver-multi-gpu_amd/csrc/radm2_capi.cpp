@@ -100,8 +100,8 @@ const radm2::KernelVariant* pick_variant(const radm2_ctx* c, bool family) {
 
 // Workspace for `waves` wave slots, grown (never shrunk) on demand.  The
 // stream is drained first: a running kernel may still use the old buffer.
-int ensure_ws(hipStream_t st, double** ws, size_t* have, int64_t waves, int ws_doubles) {
-  const size_t need = (size_t)waves * ws_doubles * radm2::WAVE * sizeof(double);
+int ensure_ws(hipStream_t st, double** ws, size_t* have, int64_t waves, size_t wave_bytes) {
+  const size_t need = (size_t)waves * wave_bytes;
   if (need <= *have) return 0;
   HIP_TRY(hipStreamSynchronize(st));
   if (*ws) HIP_TRY(hipFree(*ws));
@@ -238,6 +238,8 @@ int launch_shape(const radm2_ctx* c, int64_t total, bool family, Shape* sh, int 
   const radm2::KernelVariant* v = sh->v;
   const int block = sh->block;
   if (v->placement >= radm2::PLACE_HOT160 && block != 64) return RADM2_E_ARG;
+  // cells spread over several lanes run on the persistent schedule only
+  if (v->lanes > 1 && sched == RADM2_SCHED_GRID) return RADM2_E_UNSUPPORTED;
   const int64_t wpb = block / radm2::WAVE;
   const int64_t need = (total + block - 1) / block;
   int per_cu = 0;
@@ -285,8 +287,8 @@ int enqueue(radm2_ctx* c, const DomainDesc* d_doms, int ndom, int64_t total,
   const radm2::KernelVariant* v = sh.v;
   if (v->placement >= radm2::PLACE_WS) {
     if (grow) {
-      if (int rc = ensure_ws(st, ws, ws_bytes, sh.waves, v->ws_doubles)) return rc;
-    } else if (*ws_bytes < (size_t)sh.waves * v->ws_doubles * radm2::WAVE * sizeof(double)) {
+      if (int rc = ensure_ws(st, ws, ws_bytes, sh.waves, radm2::ws_wave_bytes(v))) return rc;
+    } else if (*ws_bytes < (size_t)sh.waves * radm2::ws_wave_bytes(v)) {
       return RADM2_E_ARG;
     }
     a.ws = *ws;
@@ -518,7 +520,7 @@ int radm2_integrate_noalloc(radm2_ctx* c, int64_t ncells, int64_t ld, double* co
   const radm2::KernelVariant* v = pick_variant(c, family);
   if (!v) return family ? RADM2_E_UNSUPPORTED : RADM2_E_ARG;
   const int64_t max_waves =
-      (int64_t)(ws_bytes / ((size_t)v->ws_doubles * radm2::WAVE * sizeof(double)));
+      (int64_t)(ws_bytes / radm2::ws_wave_bytes(v));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   HIP_TRY(radm2::launch_put_domain(tab[0], d_dom, c->stream));
   if (int rc = enqueue(c, d_dom, 1, total, counter, nullptr, c->stream, &ws, &ws_bytes, family,
@@ -564,7 +566,7 @@ int radm2_plan_create(radm2_ctx* c, int ndom, const radm2_domain* doms, radm2_pl
   if ((e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking)) != hipSuccess) return fail(e);
   // size the plan's own workspace before capture (no allocation inside it)
   if (sh.v->placement >= radm2::PLACE_WS &&
-      ensure_ws(cap, &p->d_ws, &p->ws_bytes, sh.waves, sh.v->ws_doubles) != 0) {
+      ensure_ws(cap, &p->d_ws, &p->ws_bytes, sh.waves, radm2::ws_wave_bytes(sh.v)) != 0) {
     hipStreamDestroy(cap);
     return fail(last_hip_error());
   }

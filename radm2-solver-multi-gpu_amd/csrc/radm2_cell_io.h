@@ -41,13 +41,19 @@ __device__ __forceinline__ Forcing cell_forcing(const DomainDesc& d, int64_t p) 
   return fc;
 }
 
+// writer: the lane that stores (every lane of a cell reads the state, so
+// that split entries are broadcast from their owners; one of them writes).
 template <class YA>
 __device__ __forceinline__ void cell_store(const DomainDesc& d, int64_t p, YA y,
                                            const CellState& s, double hfirst,
-                                           unsigned long long* stats) {
+                                           unsigned long long* stats, bool writer = true) {
   const int64_t ld = d.ld;
 #pragma unroll
-  for (int i = 0; i < NVAR; i++) d.conc[i * ld + p] = RD(y, i);
+  for (int i = 0; i < NVAR; i++) {
+    const double v = RD(y, i);
+    if (writer) d.conc[i * ld + p] = v;
+  }
+  if (!writer) return;
   if (d.nstep) d.nstep[p] = s.ns;
   if (d.nacc) d.nacc[p] = s.na;
   if (d.nrej) d.nrej[p] = s.nr;

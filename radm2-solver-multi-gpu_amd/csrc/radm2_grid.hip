@@ -6,6 +6,14 @@
 // the blocks that can be resident, not by the grid.
 #include "radm2_variant.h"
 
+// cells spread over several lanes (the split placements) run on the
+// persistent schedule only
+#if RADM2_PLACE == RADM2_PLACE_SPLIT2 || RADM2_PLACE == RADM2_PLACE_SPLIT4
+namespace radm2 {
+hipError_t RADM2_NAME(launch_grid)(int, const LaunchArgs&, hipStream_t) { return hipErrorNotSupported; }
+int RADM2_NAME(grid_blocks_per_cu)(int) { return -(int)hipErrorNotSupported; }
+}  // namespace radm2
+#else
 namespace radm2 {
 namespace RADM2_VARIANT_NS {
 
@@ -125,3 +133,4 @@ hipError_t RADM2_NAME(launch_grid)(int block, const LaunchArgs& a, hipStream_t s
 }
 
 }  // namespace radm2
+#endif

@@ -71,15 +71,25 @@ constexpr int PLACE_WS = RADM2_PLACE_WORKSPACE;     // explicit wave-interleaved
 // 160 doubles per lane (two blocks per CU) or 312 (one block per CU)
 constexpr int PLACE_HOT160 = RADM2_PLACE_LDS160;
 constexpr int PLACE_HOT312 = RADM2_PLACE_LDS312;
-// (LDS slots, register entries) per lane of placements 2..4; measured at 1M
-// cells: more LDS per lane (fewer resident waves) or more registers (spills)
-// or fewer LDS slots (more waves, more HBM traffic) are all slower than
-// (160, 128)
+// Hot-set configuration (LDS slots per cell, replicated register entries
+// per lane, split register entries per lane, lanes per cell) of each
+// placement; measured at 1M cells with one cell per lane: more LDS per
+// lane (fewer resident waves) or more registers (spills) or fewer LDS
+// slots (more waves, more HBM traffic) are all slower than (160, 128).
+// The split placements spread a cell over 2 or 4 lanes so that four waves
+// per CU (one per SIMD) hold as many cells (2 lanes) or half as many (4
+// lanes) as the one-lane placements' two.
 constexpr int PLACE_HOT160R128 = RADM2_PLACE_LDS160_REG128;
-constexpr int kHotConfig[5][2] = {{0, 0}, {0, 0}, {160, 0}, {312, 0}, {160, 128}};
-constexpr int PLACE_HOT_LAST = PLACE_HOT160R128;
+constexpr int PLACE_SPLIT2 = RADM2_PLACE_SPLIT2;
+constexpr int PLACE_SPLIT4 = RADM2_PLACE_SPLIT4;
+constexpr int kHotConfig[7][4] = {{0, 0, 0, 1},   {0, 0, 0, 1},     {160, 0, 0, 1},
+                                  {312, 0, 0, 1}, {160, 128, 0, 1}, {160, 0, 128, 2},
+                                  {320, 0, 128, 4}};
+constexpr int PLACE_HOT_LAST = PLACE_SPLIT4;
 constexpr int hot_slots(int placement) { return kHotConfig[placement][0]; }
 constexpr int hot_regs(int placement) { return kHotConfig[placement][1]; }
+constexpr int hot_split(int placement) { return kHotConfig[placement][2]; }
+constexpr int hot_lanes(int placement) { return kHotConfig[placement][3]; }
 
 // Kernel variants: one translation unit per (placement P, register budget
 // W = minimum resident waves per SIMD (1, 2, 4, 8), launder interval R of the
@@ -104,6 +114,8 @@ RADM2_DECLARE_VARIANT(1, 4, 0)
 RADM2_DECLARE_VARIANT(2, 1, 0)
 RADM2_DECLARE_VARIANT(3, 1, 0)
 RADM2_DECLARE_VARIANT(4, 1, 0)
+RADM2_DECLARE_VARIANT(5, 1, 0)
+RADM2_DECLARE_VARIANT(6, 1, 0)
 RADM2_DECLARE_VARIANT_NAMED(_p4_w1_r0_m1)
 #undef RADM2_DECLARE_VARIANT
 #undef RADM2_DECLARE_VARIANT_NAMED
@@ -113,13 +125,18 @@ struct KernelVariant {
   int waves;
   int launder_rows;
   int family;      // 1: KPP's five methods; 0: Ros3 only
-  int ws_doubles;  // workspace doubles per lane
+  int ws_doubles;  // workspace doubles per cell
+  int lanes;       // lanes per cell
   hipError_t (*grid)(int, const LaunchArgs&, hipStream_t);
   hipError_t (*persistent)(int, int, const LaunchArgs&, hipStream_t);
   int (*blocks_per_cu)(int);
   int (*grid_blocks_per_cu)(int);
 };
-constexpr int kNumVariants = 11;
+constexpr int kNumVariants = 13;
+// workspace bytes of one wave of a variant
+inline size_t ws_wave_bytes(const KernelVariant* v) {
+  return (size_t)v->ws_doubles * (WAVE / v->lanes) * sizeof(double);
+}
 const KernelVariant* kernel_variants();
 
 hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t st);

@@ -1,6 +1,7 @@
 // Persistent work-stealing kernel: a fixed grid of resident waves; each lane
-// integrates one cell at a time and, when its cell finishes, takes the next
-// cell index from a global counter.  Refills are wave-aggregated (one atomic
+// (each group of 2 or 4 lanes on the split placements) integrates one cell
+// at a time and, when its cell finishes, takes the next cell index from a
+// global counter.  Refills are wave-aggregated (one atomic
 // per refill round) and happen once at least `refill_min` lanes of the wave
 // are idle, trading idle lanes against the cost of the fused rate-constant
 // evaluation that a refill runs.  Cells of uneven stiffness (night cells
@@ -13,56 +14,71 @@ namespace RADM2_VARIANT_NS {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_persistent(LaunchArgs a) {
+  // lanes per cell and cells per wave (one cell per lane except on the
+  // split placements)
+  constexpr int L = hot_lanes(RADM2_PLACE), CPW = WAVE / L;
 #if RADM2_PLACE == RADM2_PLACE_PRIVATE
   double kk[NREACT], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
 #else
+  using VA = WsViewT<CPW>;
   const int64_t wslot = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE));
   constexpr int kWsDoubles = RADM2_METHODS ? WS_DOUBLES_FAMILY : WS_DOUBLES_ROS3;
-  constexpr int kWsBytes = kWsDoubles * WAVE * 8;
-  double* wbase = a.ws + wslot * ((int64_t)kWsDoubles * WAVE);
+  constexpr int kWsBytes = kWsDoubles * CPW * 8;
+  double* wbase = a.ws + wslot * ((int64_t)kWsDoubles * CPW);
   const int wlane = threadIdx.x & (WAVE - 1);
-  WsView f = make_view(wbase, WS_F, wlane, kWsBytes);
-  WsView k1 = make_view(wbase, WS_K1, wlane, kWsBytes);
-  WsView k2 = make_view(wbase, WS_K2, wlane, kWsBytes);
-  WsView k3 = make_view(wbase, WS_K3, wlane, kWsBytes);
+  VA f = make_view<CPW>(wbase, WS_F, wlane, kWsBytes);
+  VA k1 = make_view<CPW>(wbase, WS_K1, wlane, kWsBytes);
+  VA k2 = make_view<CPW>(wbase, WS_K2, wlane, kWsBytes);
+  VA k3 = make_view<CPW>(wbase, WS_K3, wlane, kWsBytes);
 #if RADM2_PLACE >= RADM2_PLACE_LDS160
-  // the hot set: kHotSlots doubles per lane in LDS, kHotRegs in registers
+  // the hot set: kHotSlots doubles per cell in LDS, kHotRegs per lane in
+  // registers, kHotSplit per lane split over the cell's lanes
   constexpr int kHotSlots = hot_slots(RADM2_PLACE), kHotRegs = hot_regs(RADM2_PLACE);
-  __shared__ double hot_lds[kHotSlots * BLOCK];
+  constexpr int kHotSplit = hot_split(RADM2_PLACE);
+  __shared__ double hot_lds[kHotSlots * CPW * (BLOCK / WAVE)];
   double hot_rg[kHotRegs > 0 ? kHotRegs : 1];
-  lds_double* wl = (lds_double*)hot_lds + (threadIdx.x / WAVE) * (kHotSlots * WAVE);
-  auto kk = make_hot<0, kHotSlots, kHotRegs>(make_view(wbase, WS_K, wlane, kWsBytes), wl, wlane, hot_rg);
-  auto y = make_hot<1, kHotSlots, kHotRegs>(make_view(wbase, WS_Y, wlane, kWsBytes), wl, wlane, hot_rg);
-  auto lu = make_hot<2, kHotSlots, kHotRegs>(make_view(wbase, WS_LU, wlane, kWsBytes), wl, wlane, hot_rg);
+  double hot_sp[kHotSplit > 0 ? kHotSplit : 1];
+  lds_double* wl = (lds_double*)hot_lds + (threadIdx.x / WAVE) * (kHotSlots * CPW);
+  auto kk = make_hot<0, kHotSlots, kHotRegs, kHotSplit, L, VA>(
+      make_view<CPW>(wbase, WS_K, wlane, kWsBytes), wl, wlane, hot_rg, hot_sp);
+  auto y = make_hot<1, kHotSlots, kHotRegs, kHotSplit, L, VA>(
+      make_view<CPW>(wbase, WS_Y, wlane, kWsBytes), wl, wlane, hot_rg, hot_sp);
+  auto lu = make_hot<2, kHotSlots, kHotRegs, kHotSplit, L, VA>(
+      make_view<CPW>(wbase, WS_LU, wlane, kWsBytes), wl, wlane, hot_rg, hot_sp);
 #else
-  WsView kk = make_view(wbase, WS_K, wlane, kWsBytes), y = make_view(wbase, WS_Y, wlane, kWsBytes);
-  WsView lu = make_view(wbase, WS_LU, wlane, kWsBytes);
+  VA kk = make_view<CPW>(wbase, WS_K, wlane, kWsBytes), y = make_view<CPW>(wbase, WS_Y, wlane, kWsBytes);
+  VA lu = make_view<CPW>(wbase, WS_LU, wlane, kWsBytes);
 #endif
 #endif
   double h2o = 0.0, m = 0.0;
   CellState s;
   s.status = ST_OK;
   s.newstep = 0;
-  int64_t cell = -1;  // global work index held by this lane, -1 when none
+  int64_t cell = -1;  // global work index held by this lane's cell, -1 when none
   int di = 0;
   bool exhausted = false;  // wave-uniform: the counter has passed the end
   const int lane = threadIdx.x & 63;
+  // the lane of its cell's group that writes the cell's results
+  const bool writer = (lane % L) == 0;
   unsigned long long refills = 0;
   for (;;) {
+    // every lane of a cell has the same state: idle lanes come in whole
+    // groups, so the idle cells number popcount / L and a cell's rank is
+    // the idle lanes below its lanes over L
     const bool idle = (s.status != ST_RUNNING);
     const unsigned long long idle_mask = __ballot(idle);
-    const int nidle = __popcll(idle_mask);
-    if (!exhausted && nidle > 0 && (nidle >= a.refill_min || nidle == 64)) {
+    const int nidle = __popcll(idle_mask) / L;
+    if (!exhausted && nidle > 0 && (nidle >= a.refill_min || nidle == CPW)) {
       if (idle && cell >= 0) {
         const DomainDesc& d = a.doms[di];
-        cell_store(d, cell - d.first, y, s, s.hfirst, a.stats);
+        cell_store(d, cell - d.first, y, s, s.hfirst, a.stats, writer);
         cell = -1;
       }
       const int leader = __ffsll((long long)idle_mask) - 1;
       unsigned long long base = 0;
       if (lane == leader) base = atomicAdd(a.counter, (unsigned long long)nidle);
       base = __shfl(base, leader);
-      const int rank = __popcll(idle_mask & ((1ull << lane) - 1ull));
+      const int rank = __popcll(idle_mask & ((1ull << lane) - 1ull)) / L;
       if (idle) {
         const int64_t gi = (int64_t)base + rank;
         if (gi < a.total) {
@@ -77,7 +93,7 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_persistent(LaunchAr
     }
     if (__ballot(s.status == ST_RUNNING) == 0ull) {
       if (exhausted) break;
-      continue;  // all idle but work remains: the next round refills all 64
+      continue;  // all idle but work remains: the next round refills every cell
     }
     if (s.status == ST_RUNNING) {
       const Ros3Opts& o = a.doms[di].opts;
@@ -93,7 +109,7 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_persistent(LaunchAr
   }
   if (cell >= 0) {
     const DomainDesc& d = a.doms[di];
-    cell_store(d, cell - d.first, y, s, s.hfirst, a.stats);
+    cell_store(d, cell - d.first, y, s, s.hfirst, a.stats, writer);
   }
   if (a.stats && lane == 0) atomicAdd(&a.stats[3], refills);
 }

@@ -37,6 +37,8 @@ PLACE_WORKSPACE = 1
 PLACE_LDS160 = 2
 PLACE_LDS312 = 3
 PLACE_LDS160_REG128 = 4
+PLACE_SPLIT2 = 5
+PLACE_SPLIT4 = 6
 
 # reference species order (mgpu.cu:125-183)
 SPECIES = (

@@ -35,36 +35,43 @@ def _array(text, name):
 
 @pytest.mark.parametrize("budget", G.HOT_BUDGETS)
 def test_hot_tables_partition_the_ranking(header, budget):
-    S, R = budget
+    S, R, P, L = budget
     ranked, acc = G.hot_ranking(MECH)
     sizes = (MECH["nreact"], MECH["nvar"], MECH["lu"]["nnz"])
     assert len(ranked) == sum(sizes) and len(set(ranked)) == len(ranked)
-    codes = [_array(header, "HOT%d_%d_%s" % (S, R, n)) for n in ("K", "Y", "G")]
-    lds, regs = [], []
+    codes = [_array(header, "%s_%s" % (G.hot_name(*budget), n)) for n in ("K", "Y", "G")]
+    lds, regs, split = [], [], []
     for a, arr in enumerate(codes):
         for i, c in enumerate(arr):
             if c >= 0:
                 lds.append((c, (a, i)))
+            elif c <= G.SPLIT_BASE:
+                split.append((G.SPLIT_BASE - c, (a, i)))
             elif c <= -2:
                 regs.append((-2 - c, (a, i)))
-    # LDS slots 0..S-1 and registers 0..R-1 each hold exactly one entry ...
+    # LDS slots 0..S-1, registers 0..R-1 and split entries 0..P*L-1 each hold
+    # exactly one entry ...
     assert sorted(s for s, _ in lds) == list(range(S))
     assert sorted(r for r, _ in regs) == list(range(R))
-    # ... the S most accessed in LDS, the next R in registers
+    assert sorted(j for j, _ in split) == list(range(P * L))
+    # ... the S most accessed in LDS, the next R in registers, the next P*L
+    # split over the cell's lanes
     assert {e for _, e in lds} == set(ranked[:S])
     assert {e for _, e in regs} == set(ranked[S:S + R])
+    assert {e for _, e in split} == set(ranked[S + R:S + R + P * L])
     # and nothing outside the tiers is accessed more than what is inside
-    inside = min(acc[e] for e in ranked[:S + R]) if S + R else 0
-    assert all(acc[e] <= inside for e in ranked[S + R:])
+    n = S + R + P * L
+    inside = min(acc[e] for e in ranked[:n]) if n else 0
+    assert all(acc[e] <= inside for e in ranked[n:])
 
 
 def test_lu_entries_are_read_after_they_are_stored(header):
     start = header.index("int jac_decomp_pf16(")
     body = header[start:header.index("\n}\n", start)]
     stored = {}
-    assert len(re.findall(r"RD\(g, \d+\)", body)) > 300   # the check is not vacuous
+    assert len(re.findall(r"RDP<\d+>\(g\)", body)) > 300   # the check is not vacuous
     for n, ln in enumerate(body.split("\n")):
-        for mo in re.finditer(r"RD\(g, (\d+)\)", ln):
+        for mo in re.finditer(r"RDP<(\d+)>\(g\)", ln):
             q = int(mo.group(1))
             assert q in stored, "LU entry %d read before its store (line %d)" % (q, n)
         mo = re.match(r"\s*WR\(g, (\d+),", ln)
@@ -80,6 +87,12 @@ def test_workspace_and_hot_accesses_use_constant_indices(header):
     # to the rate constants, the state or the LU names its element literally
     for mo in re.finditer(r"\b(RD|WR)\((k|v|g), ([^,)]+)", header):
         assert re.fullmatch(r"\d+", mo.group(3).strip()), mo.group(0)
+    # scheduled reads name theirs as a template argument
+    n = 0
+    for mo in re.finditer(r"\bRDP<([^>]+)>\((k|v|g)\)", header):
+        assert re.fullmatch(r"\d+", mo.group(1).strip()), mo.group(0)
+        n += 1
+    assert n > 1000
 
 
 def test_photolysis_map_matches_the_rate_laws():
