@@ -154,32 +154,88 @@ def photo_j(mech):
     return out
 
 
-def gen_rconst(mech):
+def rate_exprs(mech):
+    """C expression of every rate constant (reference RCONST, mgpu.cu:752-919)
+    and the shared exp/pow subexpressions they use: (exprs, pre)."""
     em = RateEmitter()
-    body = []
-    for r, e in enumerate(mech["rates"]):
+    exprs = []
+    for e in mech["rates"]:
         em._src = e
-        tree = ast.parse(e, mode="eval")
-        body.append("  WR(k, %d, %s);" % (r, em.emit(tree)))
+        exprs.append(em.emit(ast.parse(e, mode="eval")))
+    return exprs, em.pre
+
+
+def rate_forms(mech):
+    """Per-cell storage of the rate constants.  Each reaction's expression is
+    one of: a literal (not stored: written into the code at each use); A * e
+    with e one of the shared exponentials exp(c/T) (e stored once for every
+    reaction with that activation temperature, the product formed at each
+    use: the same multiplication, so the same bits); anything else -- a bare
+    J, a Troe fall-off, sums, T^2 forms, k46, the N2O5 hydrolysis input --
+    stored whole.  Returns (prims, forms): prims[p] = C expression of stored
+    value p, the 21 photolysis rate constants first (value r = reaction r,
+    the index the forcing code writes); forms[r] = ("lit", c), ("mul", A, p)
+    or ("prim", p)."""
+    import re
+    exprs, _ = rate_exprs(mech)
+    nph = len(photo_j(mech))
+    prims, index, forms = [], {}, []
+
+    def prim(key, ex):
+        if key not in index:
+            index[key] = len(prims)
+            prims.append(ex)
+        return index[key]
+    for r in range(nph):
+        assert re.fullmatch(r"J\[\d+\]", exprs[r]), exprs[r]
+        assert prim(("r", r), exprs[r]) == r
+    num = r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?"
+    for r, ex in enumerate(exprs):
+        if r < nph:
+            forms.append(("prim", r))
+        elif re.fullmatch(num, ex):
+            forms.append(("lit", ex))
+        else:
+            mo = re.fullmatch(r"\((%s) \* (rc_s\d+)\)" % num, ex)
+            if mo:
+                forms.append(("mul", mo.group(1), prim(("e", mo.group(2)), mo.group(2))))
+            else:
+                forms.append(("prim", prim(("r", r), ex)))
+    return prims, forms
+
+
+def kexpr(forms, r, rd):
+    """Code of rate constant r at a use; rd(p) reads stored value p."""
+    f = forms[r]
+    if f[0] == "lit":
+        return f[1]
+    if f[0] == "mul":
+        return "(%s * %s)" % (f[1], rd(f[2]))
+    return rd(f[1])
+
+
+def gen_rconst(mech):
+    exprs, pre = rate_exprs(mech)
+    prims, forms = rate_forms(mech)
     out = []
-    out.append("// rate constants: %d reactions, %d shared exp/pow subexpressions"
-               % (len(mech["rates"]), len(em.pre)))
+    out.append("// rate constants: %d reactions, %d shared exp/pow subexpressions; %d values"
+               " stored per cell (rate_forms)" % (len(mech["rates"]), len(pre), len(prims)))
     out.append("template <class KA>")
     out.append("__device__ __forceinline__ void rconst(double T, double M, const double* __restrict__ J,"
                " int64_t sJ, double RC, KA k) {")
     out.append("  (void)RC;")
     out.append("  const double zt = 300.0 / T;")
     # J slots are addressed with a stride
-    for name, ex in em.pre:
+    for name, ex in pre:
         out.append("  const double %s = %s;" % (name, ex))
-    for ln in body:
-        out.append(ln.replace("J[", "JJ("))
+    for p, ex in enumerate(prims):
+        out.append("  WR(k, %d, %s);" % (p, ex.replace("J[", "JJ(")))
     out.append("}")
     text = "\n".join(out)
     # JJ(i] -> J[i*sJ]
     import re
     text = re.sub(r"JJ\((\d+)\]", r"J[\1 * sJ]", text)
-    return text, len(em.pre)
+    return text, len(pre)
 
 
 # --------------------------------------------------------------------------
@@ -196,6 +252,7 @@ def gen_fun(mech):
     reaction order, so each accumulator sums in exactly the table order.
     RF > 0 launders the input views every RF reactions."""
     nterms = sum(len(t) for t in mech["tendency"])
+    _, forms = rate_forms(mech)
     out = ["// tendency: %d rates, %d terms (reaction-major scatter)" % (mech["nreact"], nterms)]
     out.append("template <int RF, class KA, class VA, class FA>")
     out.append("__device__ __forceinline__ void fun(KA k, VA v, double h2o, double m, FA f) {")
@@ -213,7 +270,7 @@ def gen_fun(mech):
             out.append("  if (RF > 0 && %d %% (RF > 0 ? RF : 1) == 0) { launder(k); launder(v); }" % r)
         if not by_r[r]:
             continue
-        expr = "RD(k, %d)" % r
+        expr = kexpr(forms, r, lambda q: "RD(k, %d)" % q)
         for kind, i in facs:
             expr = "(%s * %s)" % (expr, factor_src(kind, i))
         out.append("  { const double a = %s;" % expr)
@@ -278,10 +335,11 @@ def gen_jac_prep(mech):
     out.append("template <class KA, class VA, class GA>")
     out.append("__device__ __forceinline__ void jac_prep(KA k, VA v, double h2o, double m,"
                " double ghinv, GA g) {")
+    _, forms = rate_forms(mech)
     for p in range(659):
         acc = None
         for c, r, others in terms[p]:
-            t = "RD(k, %d)" % r
+            t = kexpr(forms, r, lambda q: "RD(k, %d)" % q)
             for kind, j in others:
                 t = "(%s * %s)" % (t, factor_src(kind, j))
             cv = float(c)
@@ -303,11 +361,11 @@ def gen_jac_prep(mech):
 # --------------------------------------------------------------------------
 # LU decomposition and solve
 # --------------------------------------------------------------------------
-def _entry_expr(terms_p):
+def _entry_expr(terms_p, forms):
     """-(sum of Jacobian terms) for one LU entry, in the contract order."""
     acc = None
     for c, r, others in terms_p:
-        t = "RD(k, %d)" % r
+        t = kexpr(forms, r, lambda q: "RD(k, %d)" % q)
         for kind, j in others:
             t = "(%s * %s)" % (t, factor_src(kind, j))
         cv = float(c)
@@ -336,6 +394,7 @@ def gen_decomp(mech, fused):
     lu = mech["lu"]
     crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
     terms = jac_terms(mech) if fused else None
+    _, forms = rate_forms(mech)
     out = []
     if fused:
         out.append("template <int R, class KA, class VA, class GA>")
@@ -355,7 +414,7 @@ def gen_decomp(mech, fused):
         for kk in range(crow[k], crow[k + 1]):
             c = icol[kk]
             if fused:
-                e = _entry_expr(terms[kk])
+                e = _entry_expr(terms[kk], forms)
                 if kk == diag[k]:
                     e = "(%s + ghinv)" % e if e else "ghinv"
                 out.append("    double w%d = %s;" % (c, e if e else "0.0"))
@@ -504,8 +563,9 @@ def schedule_loads(ops, loads, G):
 
 def gen_fun_pf(mech, G, window=16):
     """fun() with its loads scheduled like jac_decomp_pf(): the same
-    statements in the same order; rate constants are loaded once each,
-    concentrations once per window of reactions."""
+    statements in the same order; stored rate-constant values are loaded
+    once each, concentrations once per window of reactions."""
+    _, forms = rate_forms(mech)
     by_r = [[] for _ in range(mech["nreact"])]
     for i, terms in enumerate(mech["tendency"]):
         for r, c in terms:
@@ -521,8 +581,12 @@ def gen_fun_pf(mech, G, window=16):
     for r, facs in enumerate(mech["monomials"]):
         if not by_r[r]:
             continue
-        used = [need("k%d" % r, "RD(k, %d)" % r)]
-        expr = "k%d" % r
+        used = []
+
+        def rdk(q, used=used):
+            used.append(need("k%d" % q, "RD(k, %d)" % q))
+            return "k%d" % q
+        expr = kexpr(forms, r, rdk)
         for kind, i in facs:
             if kind == "V":
                 nm = "v%d_%d" % (i, r // window)
@@ -606,6 +670,7 @@ def gen_jac_decomp_pf(mech, G):
     lu = mech["lu"]
     crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
     terms = jac_terms(mech)
+    _, forms = rate_forms(mech)
     loads = []       # (name, source text, producing LU entry or None)
     load_ix = {}
     ops = []         # (statement, [load indices], stored LU entry or None)
@@ -619,7 +684,7 @@ def gen_jac_decomp_pf(mech, G):
     for k in range(59):
         for kk in range(crow[k], crow[k + 1]):
             c = icol[kk]
-            e = _entry_expr(terms[kk])
+            e = _entry_expr(terms[kk], forms)
             used = []
             if e:
                 def sub(mo, used=used):
@@ -691,7 +756,7 @@ def hot_ranking(mech):
             acc[(1, int(mo.group(1)))] += 1
     for i in range(59):
         acc[(1, i)] += 3
-    keys = [(0, r) for r in range(mech["nreact"])] + [(1, i) for i in range(59)] + \
+    keys = [(0, r) for r in range(len(rate_forms(mech)[0]))] + [(1, i) for i in range(59)] + \
         [(2, q) for q in range(lu["nnz"])]
     return sorted(keys, key=lambda key: -acc[key]), acc
 
@@ -718,7 +783,7 @@ def hot_codes(mech, S, R, P, L):
     for split entry j (the next P*L; lane j % L of the cell, register
     j // L), -1 for the HBM workspace."""
     ranked, _ = hot_ranking(mech)
-    sizes = (mech["nreact"], 59, mech["lu"]["nnz"])
+    sizes = (len(rate_forms(mech)[0]), 59, mech["lu"]["nnz"])
     code = [[-1] * n for n in sizes]
     for s_, (a, i) in enumerate(ranked[:S]):
         code[a][i] = s_
@@ -732,7 +797,7 @@ def hot_codes(mech, S, R, P, L):
 def gen_hot_tables(mech):
     """The code of every rate constant, state and LU entry per hot-set
     configuration (hot_codes), and hot_code<S, R, P, L>(array, index)."""
-    sizes = (mech["nreact"], 59, mech["lu"]["nnz"])
+    sizes = (len(rate_forms(mech)[0]), 59, mech["lu"]["nnz"])
     out = ["// hot set per (LDS slots S, replicated registers R, split registers P, lanes per"
            " cell L): code per entry (s >= 0 LDS slot s, -2 - r replicated register r,"
            " %d - j split entry j, -1 HBM)" % SPLIT_BASE]
@@ -787,6 +852,8 @@ def generate(dst=None, mech_path=None):
         "namespace gen {",
         "constexpr int NVAR = %d;" % mech["nvar"],
         "constexpr int NREACT = %d;" % mech["nreact"],
+        "// rate-constant values stored per cell (rate_forms): the working array k",
+        "constexpr int NKSTORE = %d;" % len(rate_forms(mech)[0]),
         "constexpr int LU_NNZ = %d;" % mech["lu"]["nnz"],
         "constexpr int NJ = 21;",
         "// photolysis: reaction r < NPHOTO has rate constant J[PHOTO_J[r]]",

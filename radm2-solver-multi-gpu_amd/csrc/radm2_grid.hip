@@ -31,7 +31,7 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
   const DomainDesc& d = a.doms[di];
   const int64_t p = gc - d.first;
 #if RADM2_PLACE == RADM2_PLACE_PRIVATE
-  double kk[NREACT], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
+  double kk[gen::NKSTORE], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
 #else
   // thread 0 is the block's first cell, valid whenever the block has one
   __shared__ int block_slot;

@@ -48,10 +48,12 @@ struct LaunchArgs {
   int32_t slot_words;
 };
 
-// Explicit-workspace layout per lane (doubles): rate constants, state,
+// Explicit-workspace layout per cell (doubles): the stored rate-constant
+// values (gen::NKSTORE of them: the literal rate constants are written into
+// the code, those of the form A*exp(c/T) share their exponential), state,
 // LU, four stage vectors.
 constexpr int WS_K = 0;
-constexpr int WS_Y = WS_K + NREACT;
+constexpr int WS_Y = WS_K + gen::NKSTORE;
 constexpr int WS_LU = WS_Y + NVAR;
 constexpr int WS_F = WS_LU + LU_NNZ;
 constexpr int WS_K1 = WS_F + NVAR;

@@ -18,7 +18,7 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_persistent(LaunchAr
   // split placements)
   constexpr int L = hot_lanes(RADM2_PLACE), CPW = WAVE / L;
 #if RADM2_PLACE == RADM2_PLACE_PRIVATE
-  double kk[NREACT], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
+  double kk[gen::NKSTORE], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
 #else
   using VA = WsViewT<CPW>;
   const int64_t wslot = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE));

@@ -37,7 +37,7 @@ def _array(text, name):
 def test_hot_tables_partition_the_ranking(header, budget):
     S, R, P, L = budget
     ranked, acc = G.hot_ranking(MECH)
-    sizes = (MECH["nreact"], MECH["nvar"], MECH["lu"]["nnz"])
+    sizes = (len(G.rate_forms(MECH)[0]), MECH["nvar"], MECH["lu"]["nnz"])
     assert len(ranked) == sum(sizes) and len(set(ranked)) == len(ranked)
     codes = [_array(header, "%s_%s" % (G.hot_name(*budget), n)) for n in ("K", "Y", "G")]
     lds, regs, split = [], [], []
@@ -53,7 +53,8 @@ def test_hot_tables_partition_the_ranking(header, budget):
     # exactly one entry ...
     assert sorted(s for s, _ in lds) == list(range(S))
     assert sorted(r for r, _ in regs) == list(range(R))
-    assert sorted(j for j, _ in split) == list(range(P * L))
+    nsplit = min(P * L, len(ranked) - S - R)   # the tier may outsize the arrays
+    assert sorted(j for j, _ in split) == list(range(nsplit))
     # ... the S most accessed in LDS, the next R in registers, the next P*L
     # split over the cell's lanes
     assert {e for _, e in lds} == set(ranked[:S])
@@ -93,6 +94,26 @@ def test_workspace_and_hot_accesses_use_constant_indices(header):
         assert re.fullmatch(r"\d+", mo.group(1).strip()), mo.group(0)
         n += 1
     assert n > 1000
+
+
+def test_rate_storage_forms_rebuild_every_rate_constant():
+    # each reaction's rate constant at a use is its stored value, a literal
+    # or A times a stored exponential: the same expression as the rate law
+    exprs, _ = G.rate_exprs(MECH)
+    prims, forms = G.rate_forms(MECH)
+    assert len(forms) == MECH["nreact"] and len(prims) < MECH["nreact"]
+    kinds = set()
+    for r, ex in enumerate(exprs):
+        got = G.kexpr(forms, r, lambda p: prims[p])
+        assert got == ex, (r, got, ex)
+        kinds.add(forms[r][0])
+        if forms[r][0] == "mul":
+            assert re.fullmatch(r"rc_s\d+", prims[forms[r][2]])
+    assert kinds == {"lit", "mul", "prim"}
+    # the photolysis rate constants keep their reaction's index (the forcing
+    # code writes them there)
+    for r in range(len(G.photo_j(MECH))):
+        assert forms[r] == ("prim", r)
 
 
 def test_photolysis_map_matches_the_rate_laws():
