@@ -33,11 +33,19 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
 #if RADM2_PLACE == RADM2_PLACE_PRIVATE
   double kk[gen::NKSTORE], y[NVAR], lu[LU_NNZ], f[NVAR], k1[NVAR], k2[NVAR], k3[NVAR];
 #else
-  // thread 0 is the block's first cell, valid whenever the block has one
-  __shared__ int block_slot;
-  if (threadIdx.x == 0) block_slot = acquire_slot(a.slot_bits, a.slot_words, (int)blockIdx.x);
-  __syncthreads();
-  const int slot = block_slot;
+  // thread 0 is the block's first cell, valid whenever the block has one;
+  // a one-wave block passes the slot on through the wave (no LDS word: the
+  // LDS placements' blocks fill their share of the CU's LDS exactly)
+  int slot = 0;
+  if constexpr (BLOCK == WAVE) {
+    if (threadIdx.x == 0) slot = acquire_slot(a.slot_bits, a.slot_words, (int)blockIdx.x);
+    slot = __builtin_amdgcn_readfirstlane(slot);
+  } else {
+    __shared__ int block_slot;
+    if (threadIdx.x == 0) block_slot = acquire_slot(a.slot_bits, a.slot_words, (int)blockIdx.x);
+    __syncthreads();
+    slot = block_slot;
+  }
   const int64_t wslot =
       __builtin_amdgcn_readfirstlane((int)((int64_t)slot * (BLOCK / WAVE) + threadIdx.x / WAVE));
   constexpr int kWsDoubles = RADM2_METHODS ? WS_DOUBLES_FAMILY : WS_DOUBLES_ROS3;
@@ -89,7 +97,7 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
 #if RADM2_PLACE != RADM2_PLACE_PRIVATE
   // every wave of the block is done with the slot's workspace
   __threadfence();
-  __syncthreads();
+  if constexpr (BLOCK != WAVE) __syncthreads();
   if (threadIdx.x == 0) release_slot(a.slot_bits, slot);
 #endif
 }

@@ -41,48 +41,53 @@ def build():
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
 
+def load(path):
+    """The oracle library at `path` with its functions' signatures bound."""
+    L = ctypes.CDLL(path)
+    L.oracle_rconst.argtypes = [ctypes.c_double, ctypes.c_double, _D, ctypes.c_double, _D]
+    L.oracle_default_rc_n2o5.restype = ctypes.c_double
+    L.oracle_fun.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D, _D]
+    L.oracle_jac.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D, _D]
+    L.oracle_jac_prep.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D, ctypes.c_double, _D]
+    L.oracle_decomp.argtypes = [_D, ctypes.c_int]
+    L.oracle_decomp.restype = ctypes.c_int
+    L.oracle_solve.argtypes = [_D, _D, ctypes.c_int]
+    L.oracle_resolve.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.POINTER(OptsIn), ctypes.POINTER(OptsOut)]
+    L.oracle_resolve.restype = ctypes.c_int
+    L.oracle_ros3_cell.argtypes = [_D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_double, ctypes.POINTER(OptsOut),
+                                   ctypes.c_double, _I, _I, _I, _D]
+    L.oracle_ros3_cell.restype = ctypes.c_int
+    L.oracle_ros_cell.argtypes = ([ctypes.c_int, _D, _D, _D, _D]
+                                  + L.oracle_ros3_cell.argtypes[2:])
+    L.oracle_ros_cell.restype = ctypes.c_int
+    L.oracle_method_coeffs.argtypes = [ctypes.c_int] + [_D] * 7
+    L.oracle_method_coeffs.restype = None
+    L.oracle_photo_at.argtypes = [_D, _D, _D, ctypes.c_double, ctypes.c_double]
+    L.oracle_step_control.argtypes = [ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(OptsOut), _D, _D, _I, _I]
+    L.oracle_step_control.restype = ctypes.c_int
+    L.oracle_ros_attempt.argtypes = [ctypes.c_int, _D, _D] + [ctypes.c_double] * 5 + [_D, _D]
+    L.oracle_ros_attempt.restype = ctypes.c_double
+    L.oracle_ros_attempt_tab.argtypes = ([ctypes.c_int, _D, _D, _D, _D, _I, ctypes.c_double,
+                                          _D, _D] + [ctypes.c_double] * 5 + [_D, _D])
+    L.oracle_ros_attempt_tab.restype = ctypes.c_double
+    L.oracle_photo_at.restype = None
+    L.oracle_integrate.argtypes = [ctypes.c_int64, ctypes.c_int64, _D, _D, _D, _D, _D, _D, _D,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.POINTER(OptsIn), _D, _I, _I, _I, _I, _D,
+                                   ctypes.c_int]
+    L.oracle_integrate.restype = ctypes.c_int
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = ctypes.CDLL(LIB)
-        L.oracle_rconst.argtypes = [ctypes.c_double, ctypes.c_double, _D, ctypes.c_double, _D]
-        L.oracle_default_rc_n2o5.restype = ctypes.c_double
-        L.oracle_fun.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D, _D]
-        L.oracle_jac.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D, _D]
-        L.oracle_jac_prep.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D, ctypes.c_double, _D]
-        L.oracle_decomp.argtypes = [_D, ctypes.c_int]
-        L.oracle_decomp.restype = ctypes.c_int
-        L.oracle_solve.argtypes = [_D, _D, ctypes.c_int]
-        L.oracle_resolve.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                     ctypes.POINTER(OptsIn), ctypes.POINTER(OptsOut)]
-        L.oracle_resolve.restype = ctypes.c_int
-        L.oracle_ros3_cell.argtypes = [_D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                       ctypes.c_double, ctypes.c_double, ctypes.POINTER(OptsOut),
-                                       ctypes.c_double, _I, _I, _I, _D]
-        L.oracle_ros3_cell.restype = ctypes.c_int
-        L.oracle_ros_cell.argtypes = ([ctypes.c_int, _D, _D, _D, _D]
-                                      + L.oracle_ros3_cell.argtypes[2:])
-        L.oracle_ros_cell.restype = ctypes.c_int
-        L.oracle_method_coeffs.argtypes = [ctypes.c_int] + [_D] * 7
-        L.oracle_method_coeffs.restype = None
-        L.oracle_photo_at.argtypes = [_D, _D, _D, ctypes.c_double, ctypes.c_double]
-        L.oracle_step_control.argtypes = [ctypes.c_double, ctypes.c_double,
-                                          ctypes.POINTER(OptsOut), _D, _D, _I, _I]
-        L.oracle_step_control.restype = ctypes.c_int
-        L.oracle_ros_attempt.argtypes = [ctypes.c_int, _D, _D] + [ctypes.c_double] * 5 + [_D, _D]
-        L.oracle_ros_attempt.restype = ctypes.c_double
-        L.oracle_ros_attempt_tab.argtypes = ([ctypes.c_int, _D, _D, _D, _D, _I, ctypes.c_double,
-                                              _D, _D] + [ctypes.c_double] * 5 + [_D, _D])
-        L.oracle_ros_attempt_tab.restype = ctypes.c_double
-        L.oracle_photo_at.restype = None
-        L.oracle_integrate.argtypes = [ctypes.c_int64, ctypes.c_int64, _D, _D, _D, _D, _D, _D, _D,
-                                       ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                       ctypes.POINTER(OptsIn), _D, _I, _I, _I, _I, _D,
-                                       ctypes.c_int]
-        L.oracle_integrate.restype = ctypes.c_int
-        _lib = L
+        _lib = load(LIB)
     return _lib
 
 
@@ -176,9 +181,10 @@ def ros_attempt(method, y, k, h2o, m, H, rtol=1e-3, atol=1.0, table=None):
 
 
 def integrate(cells, dt, rtol, atol, opts=None, h0=None, nthreads=0, n=None, use_rc=True,
-              jrate_end=None):
+              jrate_end=None, library=None):
     """Integrate make_cells()-style host arrays; returns (conc, counters dict, code).
-    jrate_end: J rates at the end of the interval (non-autonomous), or None."""
+    jrate_end: J rates at the end of the interval (non-autonomous), or None.
+    library: an oracle library from load() (default: the test build)."""
     conc = np.array(cells["conc"], dtype=np.float64, order="C")
     ld = conc.shape[1]
     n = ld if n is None else n
@@ -190,7 +196,7 @@ def integrate(cells, dt, rtol, atol, opts=None, h0=None, nthreads=0, n=None, use
     rc_arr = np.ascontiguousarray(cells["rc_n2o5"]) if use_rc else None
     h0a = None if h0 is None else np.ascontiguousarray(h0, dtype=np.float64)
     j1 = None if jrate_end is None else np.ascontiguousarray(jrate_end, dtype=np.float64)
-    code = lib().oracle_integrate(
+    code = (library or lib()).oracle_integrate(
         n, ld, _p(conc), _p(np.ascontiguousarray(cells["temp"])),
         _p(np.ascontiguousarray(cells["cair"])), _p(np.ascontiguousarray(cells["h2o"])),
         _p(np.ascontiguousarray(cells["jrate"])), _p(j1), _p(rc_arr), dt, rtol, atol,

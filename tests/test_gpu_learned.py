@@ -49,6 +49,48 @@ def test_learned_h0_gain_and_accuracy(solver):
     assert e_learned <= 1.25 * e_plain
 
 
+def _oracle_ref(seed):
+    """The CPU oracle's rtol-1e-10 end states of the first cells of a
+    1,048,576-cell workload (tests/golden/accuracy_ref.npz) and the digest of
+    their initial states."""
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "accuracy_ref.npz"))
+    return d["s%d_conc" % seed], str(d["s%d_conc0_sha256" % seed])
+
+
+def _cell_err(c, ref):
+    big = np.abs(ref) > 1e3
+    return np.where(big, np.abs(c - ref) / np.maximum(np.abs(ref), 1e-300), 0.0).max(0)
+
+
+def test_shipped_network_on_the_held_out_million(solver):
+    # the workload as specified: the shipped network (trained on 2,097,152
+    # seed-1 cells) on all 1,048,576 held-out seed-2 cells, proposals
+    # clamped to [1e-6 s, dt]; every cell succeeds, attempts fall, and on
+    # the first cells the error against the CPU oracle at rtol 1e-10 is no
+    # worse than the plain call's
+    import hashlib
+    from radm2_amd import learned_h0, synth
+    path = os.path.join(os.path.dirname(learned_h0.__file__), "..", "mech", "h0_mlp.npz")
+    net, mu, sd = learned_h0.load(path)
+    held = synth.make_cells(1048576, 2, 100.0, 0.05)
+    ref, digest = _oracle_ref(2)
+    m = ref.shape[1]
+    assert hashlib.sha256(np.ascontiguousarray(held["conc"][:, :m]).tobytes()).hexdigest() == digest
+    r = learned_h0.evaluate(solver, net, mu, sd, held, reps=0, hlo=1e-6)
+    plain, learned = _check(r, 1048576, "held-out million")
+    e_plain = _cell_err(plain["conc"][:, :m], ref)
+    e_learned = _cell_err(learned["conc"][:, :m], ref)
+    print("held-out 1M cells: attempts %.2f -> %.2f, rejected %.2f -> %.2f, used %.3f; first %d "
+          "cells vs rtol-1e-10 oracle: max %.3e -> %.3e, median %.3e -> %.3e"
+          % (plain["attempts"], learned["attempts"], plain["rejected"], learned["rejected"],
+             learned["used"], m, e_plain.max(), e_learned.max(), np.median(e_plain),
+             np.median(e_learned)))
+    assert learned["attempts"] < plain["attempts"]
+    assert learned["rejected"] < plain["rejected"]
+    assert e_learned.max() <= 1.25 * e_plain.max()
+    assert np.median(e_learned) <= 1.25 * np.median(e_plain)
+
+
 def test_shipped_network_on_unseen_cells(solver):
     from radm2_amd import learned_h0, synth
     path = os.path.join(os.path.dirname(learned_h0.__file__), "..", "mech", "h0_mlp.npz")

@@ -351,6 +351,44 @@ def test_batched_64_uneven_domains_plan_matches_oracle(solver):
     plan.close()
 
 
+def test_autotuner_picks_a_compiled_configuration(solver, cells):
+    # coordinate descent over placement x register budget, schedule x block
+    # and refill on a sample: every candidate the library accepts is timed,
+    # every one gives the first candidate's end states and step counts bit
+    # for bit, the batch comes back restored, and the pick runs
+    import json
+    import torch
+    import radm2_amd
+    from radm2_amd import autotune
+    sub = {k: np.ascontiguousarray(v[..., :2048]) for k, v in cells.items()
+           if k in ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")}
+    b = _batch(sub)
+    c0 = b.conc.clone()
+    try:
+        r = autotune.autotune(solver, b, 600.0, sample=1024, cache=None, use_cache=False)
+        assert r["mismatches"] == [] and r["sample"] == 1024 and not r["cached"]
+        assert torch.equal(b.conc, c0)
+        finite = {k: v for k, v in r["timings"].items() if np.isfinite(v)}
+        assert json.dumps(r["best"], sort_keys=True) in finite
+        assert len(finite) >= 15
+        # every kernel variant and every schedule of the search was timed
+        seen = [json.loads(k) for k in finite]
+        assert {(c["placement"], c["waves"]) for c in seen} >= set(autotune.KERNELS)
+        assert {(c["schedule"], c["block"]) for c in seen} >= {
+            (radm2_amd.SCHED_PERSISTENT, 64), (radm2_amd.SCHED_GRID, 64)}
+        print("autotune best %s of %d candidates" % (r["best"], len(finite)))
+        # the chosen configuration is applied and integrates like the oracle
+        solver.integrate(b, 600.0, 1e-3, 1.0)
+        solver.sync()
+        out = _out(b)
+        co, oo, code = O.integrate(sub, 600.0, 1e-3, 1.0)
+        assert code == 0 and _close(out["conc"], co).all()
+        assert (out["nstep"] == oo["nstep"]).all()
+    finally:
+        solver.set_schedule(1, 0, 0)
+        solver.set_kernel(-1, 0)
+
+
 def test_measured_fp64_peak_is_plausible(solver):
     # the roofline report's attainable FP64 peak (FMA chains in registers):
     # positive and not above MI355X's published 78.6 TFLOP/s by more than

@@ -135,3 +135,38 @@ def test_surrogate_error_and_fallback_report(solver, trained):
           % (s["fallback_rate"], s["resolved"], f.size, err(p["conc"], slice(None)),
              err(s["conc"], slice(None)), err(s["conc"], pred), pred.sum()))
     assert 0.05 <= s["fallback_rate"] <= 1.0
+
+
+def test_shipped_surrogate_on_the_million(solver):
+    # the workload as specified: the shipped surrogate on 1,048,576 seed-3
+    # cells at the 1e-3 gate with a 5% audit; re-solved cells are the plain
+    # solver's bit for bit, the fallback rate and the error against the CPU
+    # oracle at rtol 1e-10 (first cells) are reported
+    import hashlib
+    import os
+    from radm2_amd import surrogate, synth
+    net, norm = surrogate.load(os.path.join(os.path.dirname(surrogate.__file__), "..", "mech",
+                                            "surrogate_mlp.npz"))
+    ev = synth.make_cells(1048576, 3, 100.0, 0.05)
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "accuracy_ref.npz"))
+    ref = d["s3_conc"]
+    m = ref.shape[1]
+    assert (hashlib.sha256(np.ascontiguousarray(ev["conc"][:, :m]).tobytes()).hexdigest()
+            == str(d["s3_conc0_sha256"]))
+    r = surrogate.evaluate(solver, net, norm, ev, thr=1e-3, audit=0.05, reps=0)
+    s, p = r["surrogate"], r["plain"]
+    f = s["flag"].astype(bool)
+    assert s["resolved"] == f.sum() and s["ok_resolved"] == 1048576
+    np.testing.assert_array_equal(s["conc"][:, f], p["conc"][:, f])
+    big = np.abs(ref) > 1e3
+
+    def err(c):
+        e = np.where(big, np.abs(c[:, :m] - ref) / np.maximum(np.abs(ref), 1e-300), 0.0).max(0)
+        return e
+    ep, es = err(p["conc"]), err(s["conc"])
+    print("surrogate on 1M cells: fallback rate %.4f, plain %.3f s, pass %.3f s; first %d cells "
+          "vs rtol-1e-10 oracle: plain max %.3e, pass max %.3e"
+          % (s["fallback_rate"], p["seconds"], s["seconds"], m, ep.max(), es.max()))
+    assert 0.05 <= s["fallback_rate"] <= 1.0
+    # where the judge re-solved, the pass carries the plain solve's error
+    np.testing.assert_array_equal(es[f[:m]], ep[f[:m]])
