@@ -1,17 +1,22 @@
 // Small-MLP kernels for the learned step-size proposal and end-state
-// surrogate: BF16 operands on MFMA (v_mfma_f32_16x16x16_bf16, wave64) with
-// FP32 accumulation, plus the elementwise pieces of training (activation
-// derivative, loss gradient, Adam) and the cell feature extraction.
-// No reference counterpart (the reference has no learned parts).
+// surrogate: BF16 operands on gfx950's MFMA (v_mfma_f32_16x16x32_bf16, wave64)
+// with FP32 accumulation, plus the elementwise pieces of training
+// (activation derivative, loss gradient, Adam) and the cell feature
+// extraction.  No reference counterpart (the reference has no learned parts).
 //
-// Matrices are row-major.  Every dimension the GEMMs see is a multiple of
-// 16 (the host pads features to 96 and outputs to 16); each wave computes
-// one 16x16 output tile, walking the reduction dimension 16 at a time.
+// Matrices are row-major.  The three products of a dense layer (forward,
+// data gradient, weight gradient) run through one tiled kernel: a block of
+// four waves computes a 64 x 64 output tile, staging 64 x 32 slices of both
+// operands in LDS per step of the reduction (coalesced global loads along
+// whichever index is contiguous, zero-filled outside the matrices), each
+// wave a 32 x 32 quarter as 2 x 2 MFMA tiles.  Both operands sit in LDS
+// with the reduction index contiguous, so a lane's fragment is one 16-byte
+// read.
 //
-// MFMA 16x16x16 bf16 fragment layout (wave64), lane l, item j = 0..3:
-//   A (16x16, rows x red):  A[l % 16][4*(l/16) + j]
-//   B (16x16, red x cols):  B[4*(l/16) + j][l % 16]
-//   D (16x16, rows x cols): D[4*(l/16) + j][l % 16]
+// MFMA 16x16x32 bf16 fragment layout (wave64), lane l:
+//   A (16x32, rows x red):  A[l % 16][8*(l/16) + j], j = 0..7
+//   B (32x16, red x cols):  B[8*(l/16) + j][l % 16], j = 0..7
+//   D (16x16, rows x cols): D[4*(l/16) + j][l % 16], j = 0..3
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -20,7 +25,7 @@
 namespace radm2 {
 namespace mlp {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bf16 bits, round to nearest even (finite inputs)
@@ -36,34 +41,87 @@ __device__ __forceinline__ float silu_grad(float z) {
   return s * (1.0f + z * (1.0f - s));
 }
 
-// Y[M x N] = act(X[M x K] W[K x N] + b); Z keeps the pre-activation (FP32)
-// for the backward pass; Y in BF16 (next layer's operand) and/or FP32.
-__global__ __launch_bounds__(64) void k_dense_fwd(int M, int K, int N, const uint16_t* __restrict__ X,
-                                                  const uint16_t* __restrict__ W,
-                                                  const float* __restrict__ b, int act,
-                                                  float* __restrict__ Z, uint16_t* __restrict__ Yb,
-                                                  float* __restrict__ Yf) {
-  const int lane = threadIdx.x;
-  const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
-  const int r = lane & 15, q = lane >> 4;
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    bf16x4 a, bb;
-    for (int j = 0; j < 4; j++) {
-      a[j] = (short)X[(size_t)(m0 + r) * K + k0 + 4 * q + j];
-      bb[j] = (short)W[(size_t)(k0 + 4 * q + j) * N + n0 + r];
+// C[M x N] = sum_r A(i, r) B(r, j) over r in [z*rchunk, min(R, (z+1)*rchunk)),
+// with A(i, r) = A[i*ai + r*ar] and B(r, j) = B[r*br + j*bj].  Epilogues:
+//   EPI_FWD     z = C + b, Z = z (FP32), Y = act(z) as BF16 (Yb) and/or FP32 (Yf)
+//   EPI_STORE   C[i*ldc + j] = sum
+//   EPI_ATOMIC  C[i*ldc + j] += sum (one FP32 atomic per element per chunk)
+constexpr int GBM = 64, GBN = 64, GBK = 32, GPAD = 8;
+enum { EPI_FWD = 0, EPI_STORE = 1, EPI_ATOMIC = 2 };
+
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int R, int rchunk,
+                                              const uint16_t* __restrict__ A, int64_t ai, int64_t ar,
+                                              const uint16_t* __restrict__ B, int64_t br, int64_t bj,
+                                              float* __restrict__ C, int64_t ldc,
+                                              const float* __restrict__ bias, int act,
+                                              float* __restrict__ Z, uint16_t* __restrict__ Yb,
+                                              float* __restrict__ Yf) {
+  // reduction index contiguous in both tiles: A as [i][r], B as [j][r]
+  __shared__ __attribute__((aligned(16))) uint16_t As[GBM][GBK + GPAD];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[GBN][GBK + GPAD];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int i0 = blockIdx.x * GBM, j0 = blockIdx.y * GBN;
+  const int rb = blockIdx.z * rchunk, re = min(R, rb + rchunk);
+  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int s = 0; s < 2; s++)
+#pragma unroll
+    for (int u = 0; u < 2; u++) acc[s][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int r0 = rb; r0 < re; r0 += GBK) {
+#pragma unroll
+    for (int e = t; e < GBM * GBK; e += 256) {
+      const int ii = ar == 1 ? e / GBK : e % GBM, rr = ar == 1 ? e % GBK : e / GBM;
+      const int gi = i0 + ii, gr = r0 + rr;
+      As[ii][rr] = (gi < M && gr < re) ? A[(int64_t)gi * ai + (int64_t)gr * ar] : (uint16_t)0;
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc, 0, 0, 0);
+#pragma unroll
+    for (int e = t; e < GBN * GBK; e += 256) {
+      const int jj = bj == 1 ? e % GBN : e / GBK, rr = bj == 1 ? e / GBN : e % GBK;
+      const int gj = j0 + jj, gr = r0 + rr;
+      Bs[jj][rr] = (gj < N && gr < re) ? B[(int64_t)gr * br + (int64_t)gj * bj] : (uint16_t)0;
+    }
+    __syncthreads();
+    bf16x8 fa[2], fb[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      fa[s] = *reinterpret_cast<const bf16x8*>(&As[wm + 16 * s + (lane & 15)][8 * (lane >> 4)]);
+      fb[s] = *reinterpret_cast<const bf16x8*>(&Bs[wn + 16 * s + (lane & 15)][8 * (lane >> 4)]);
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+        acc[s][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[s], fb[u], acc[s][u], 0, 0, 0);
+    __syncthreads();
   }
-  for (int j = 0; j < 4; j++) {
-    const int m = m0 + 4 * q + j, n = n0 + r;
-    const float z = acc[j] + (b ? b[n] : 0.0f);
-    const float y = act == RADM2_ACT_SILU ? silu(z) : z;
-    const size_t o = (size_t)m * N + n;
-    if (Z) Z[o] = z;
-    if (Yb) Yb[o] = f2bf(y);
-    if (Yf) Yf[o] = y;
-  }
+#pragma unroll
+  for (int s = 0; s < 2; s++)
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int i = i0 + wm + 16 * s + 4 * (lane >> 4) + q, j = j0 + wn + 16 * u + (lane & 15);
+        if (i >= M || j >= N) continue;
+        const float v = acc[s][u][q];
+        const int64_t o = (int64_t)i * ldc + j;
+        if constexpr (EPI == EPI_FWD) {
+          const float z = v + (bias ? bias[j] : 0.0f);
+          const float y = act == RADM2_ACT_SILU ? silu(z) : z;
+          if (Z) Z[o] = z;
+          if (Yb) Yb[o] = f2bf(y);
+          if (Yf) Yf[o] = y;
+        } else if constexpr (EPI == EPI_STORE) {
+          C[o] = v;
+        } else {
+          atomicAdd(&C[o], v);
+        }
+      }
+}
+
+static dim3 gemm_grid(int M, int N, int z) {
+  return dim3((unsigned)((M + GBM - 1) / GBM), (unsigned)((N + GBN - 1) / GBN), (unsigned)z);
 }
 
 // dZ = dY * act'(Z) as BF16 (the backward GEMMs' operand); FP32 copy kept
@@ -75,48 +133,6 @@ __global__ void k_act_bwd(int64_t n, const float* __restrict__ dY, const float* 
   const float g = act == RADM2_ACT_SILU ? dY[i] * silu_grad(Z[i]) : dY[i];
   dZb[i] = f2bf(g);
   dZf[i] = g;
-}
-
-// dX[M x K] = dZ[M x N] W^T  (W is K x N)
-__global__ __launch_bounds__(64) void k_dense_bwd_data(int M, int K, int N,
-                                                       const uint16_t* __restrict__ dZ,
-                                                       const uint16_t* __restrict__ W,
-                                                       float* __restrict__ dX) {
-  const int lane = threadIdx.x;
-  const int m0 = blockIdx.x * 16, k0 = blockIdx.y * 16;
-  const int r = lane & 15, q = lane >> 4;
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int n1 = 0; n1 < N; n1 += 16) {
-    bf16x4 a, bb;
-    for (int j = 0; j < 4; j++) {
-      a[j] = (short)dZ[(size_t)(m0 + r) * N + n1 + 4 * q + j];       // A[m][n]
-      bb[j] = (short)W[(size_t)(k0 + r) * N + n1 + 4 * q + j];       // B[n][k] = W[k][n]
-    }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc, 0, 0, 0);
-  }
-  for (int j = 0; j < 4; j++) dX[(size_t)(m0 + 4 * q + j) * K + k0 + r] = acc[j];
-}
-
-// dW[K x N] += X^T dZ over rows [mb*chunk, (mb+1)*chunk) of the batch, one
-// FP32 atomic add per element per chunk.
-__global__ __launch_bounds__(64) void k_dense_bwd_weight(int M, int K, int N, int chunk,
-                                                         const uint16_t* __restrict__ X,
-                                                         const uint16_t* __restrict__ dZ,
-                                                         float* __restrict__ dW) {
-  const int lane = threadIdx.x;
-  const int k0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
-  const int mb = blockIdx.z * chunk, me = min(M, mb + chunk);
-  const int r = lane & 15, q = lane >> 4;
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int m1 = mb; m1 < me; m1 += 16) {
-    bf16x4 a, bb;
-    for (int j = 0; j < 4; j++) {
-      a[j] = (short)X[(size_t)(m1 + 4 * q + j) * K + k0 + r];        // A[k][m] = X[m][k]
-      bb[j] = (short)dZ[(size_t)(m1 + 4 * q + j) * N + n0 + r];      // B[m][n]
-    }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc, 0, 0, 0);
-  }
-  for (int j = 0; j < 4; j++) atomicAdd(&dW[(size_t)(k0 + 4 * q + j) * N + n0 + r], acc[j]);
 }
 
 // db[N] += sum over rows of dZ (FP32)
@@ -355,8 +371,10 @@ extern "C" {
 int radm2_mlp_dense_fwd(void* stream, int M, int K, int N, const uint16_t* X, const uint16_t* W,
                         const float* b, int act, float* Z, uint16_t* Yb, float* Yf) {
   if (M <= 0 || K <= 0 || N <= 0 || (M | K | N) & 15 || !X || !W) return RADM2_E_ARG;
-  hipLaunchKernelGGL(k_dense_fwd, dim3(M / 16, N / 16), dim3(64), 0, (hipStream_t)stream, M, K, N,
-                     X, W, b, act, Z, Yb, Yf);
+  // Y[M x N] = act(X[M x K] W[K x N] + b)
+  hipLaunchKernelGGL(k_gemm<EPI_FWD>, gemm_grid(M, N, 1), dim3(256), 0, (hipStream_t)stream, M, N, K,
+                     K, X, (int64_t)K, (int64_t)1, W, (int64_t)N, (int64_t)1, nullptr, (int64_t)N, b,
+                     act, Z, Yb, Yf);
   return launch_ok();
 }
 
@@ -372,11 +390,18 @@ int radm2_mlp_dense_bwd(void* stream, int M, int K, int N, const uint16_t* X, co
                         const uint16_t* dZb, const float* dZf, float* dX, float* dW, float* db) {
   if (M <= 0 || K <= 0 || N <= 0 || (M | K | N) & 15 || !X || !W || !dZb) return RADM2_E_ARG;
   hipStream_t st = (hipStream_t)stream;
-  if (dX) hipLaunchKernelGGL(k_dense_bwd_data, dim3(M / 16, K / 16), dim3(64), 0, st, M, K, N, dZb, W, dX);
+  // dX[M x K] = dZ[M x N] W^T (W is K x N)
+  if (dX)
+    hipLaunchKernelGGL(k_gemm<EPI_STORE>, gemm_grid(M, K, 1), dim3(256), 0, st, M, K, N, N, dZb,
+                       (int64_t)N, (int64_t)1, W, (int64_t)1, (int64_t)N, dX, (int64_t)K, nullptr, 0,
+                       nullptr, nullptr, nullptr);
+  // dW[K x N] += X^T dZ, the batch rows in chunks of 1024 (one atomic add
+  // per element per chunk)
   if (dW) {
     const int chunk = 1024;
-    hipLaunchKernelGGL(k_dense_bwd_weight, dim3(K / 16, N / 16, (M + chunk - 1) / chunk), dim3(64), 0,
-                       st, M, K, N, chunk, X, dZb, dW);
+    hipLaunchKernelGGL(k_gemm<EPI_ATOMIC>, gemm_grid(K, N, (M + chunk - 1) / chunk), dim3(256), 0, st,
+                       K, N, M, chunk, X, (int64_t)1, (int64_t)K, dZb, (int64_t)N, (int64_t)1, dW,
+                       (int64_t)N, nullptr, 0, nullptr, nullptr, nullptr);
   }
   if (db) {
     if (!dZf) return RADM2_E_ARG;
