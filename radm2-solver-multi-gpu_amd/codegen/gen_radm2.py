@@ -768,8 +768,11 @@ def hot_ranking(mech):
 # per lane
 HOT_BUDGETS = ((160, 0, 0, 1), (312, 0, 0, 1), (160, 128, 0, 1), (160, 0, 128, 2),
                (320, 0, 128, 4))
-# code of a split-tier entry: SPLIT_BASE - (its index in the split tier)
+# code of a split-tier entry: SPLIT_BASE - (its index in the split tier);
+# of a workspace entry: HBM_BASE - (its rank among the workspace entries,
+# 0 the most accessed)
 SPLIT_BASE = -(1 << 14)
+HBM_BASE = -(1 << 13)
 
 
 def hot_name(S, R, P, L):
@@ -781,16 +784,20 @@ def hot_codes(mech, S, R, P, L):
     configuration (S, R, P, L): s >= 0 for LDS slot s (the first S of the
     ranking), -2 - r for replicated register r (the next R), SPLIT_BASE - j
     for split entry j (the next P*L; lane j % L of the cell, register
-    j // L), -1 for the HBM workspace."""
+    j // L), HBM_BASE - h for the h-th most accessed of the entries left in
+    the HBM workspace."""
     ranked, _ = hot_ranking(mech)
     sizes = (len(rate_forms(mech)[0]), 59, mech["lu"]["nnz"])
-    code = [[-1] * n for n in sizes]
+    code = [[None] * n for n in sizes]
     for s_, (a, i) in enumerate(ranked[:S]):
         code[a][i] = s_
     for r, (a, i) in enumerate(ranked[S:S + R]):
         code[a][i] = -2 - r
     for j, (a, i) in enumerate(ranked[S + R:S + R + P * L]):
         code[a][i] = SPLIT_BASE - j
+    for h, (a, i) in enumerate(ranked[S + R + P * L:]):
+        code[a][i] = HBM_BASE - h
+    assert all(c is not None for arr in code for c in arr)
     return code
 
 
@@ -800,8 +807,10 @@ def gen_hot_tables(mech):
     sizes = (len(rate_forms(mech)[0]), 59, mech["lu"]["nnz"])
     out = ["// hot set per (LDS slots S, replicated registers R, split registers P, lanes per"
            " cell L): code per entry (s >= 0 LDS slot s, -2 - r replicated register r,"
-           " %d - j split entry j, -1 HBM)" % SPLIT_BASE]
+           " %d - j split entry j, %d - h the h-th most accessed HBM entry)"
+           % (SPLIT_BASE, HBM_BASE)]
     out.append("constexpr int HOT_SPLIT_BASE = %d;" % SPLIT_BASE)
+    out.append("constexpr int HOT_HBM_BASE = %d;" % HBM_BASE)
     for S, R, P, L in HOT_BUDGETS:
         code = hot_codes(mech, S, R, P, L)
         for a, name in enumerate(("K", "Y", "G")):

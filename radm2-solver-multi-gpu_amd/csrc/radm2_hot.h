@@ -86,6 +86,13 @@ __device__ __forceinline__ HotView<ARR, S, R, P, L, WS> make_hot(const WS& ws, l
   return v;
 }
 
+// Workspace entries at or beyond rank RADM2_WARM among the workspace's
+// entries are accessed with the streaming cache hint, so that the L2 keeps
+// the more accessed ones.
+#ifndef RADM2_WARM
+#define RADM2_WARM (1 << 30)
+#endif
+
 template <int ARR, int S, int R, int P, int L, class WS>
 __device__ __forceinline__ double RD(const HotView<ARR, S, R, P, L, WS>& v, int i) {
   const int c = gen::hot_code<S, R, P, L>(ARR, i);
@@ -98,8 +105,11 @@ __device__ __forceinline__ double RD(const HotView<ARR, S, R, P, L, WS>& v, int 
       return x;
     }
   }
-  if (c <= -2) return v.regs[-2 - c];
-  return RD(v.ws, i);
+  if (c <= gen::HOT_HBM_BASE) {
+    if (gen::HOT_HBM_BASE - c >= RADM2_WARM) return RD_stream(v.ws, i);
+    return RD(v.ws, i);
+  }
+  return v.regs[-2 - c];
 }
 template <int ARR, int S, int R, int P, int L, class WS>
 __device__ __forceinline__ void WR(const HotView<ARR, S, R, P, L, WS>& v, int i, double x) {
@@ -116,10 +126,12 @@ __device__ __forceinline__ void WR(const HotView<ARR, S, R, P, L, WS>& v, int i,
       return;
     }
   }
-  if (c <= -2)
-    v.regs[-2 - c] = x;
-  else
-    WR(v.ws, i, x);
+  if (c <= gen::HOT_HBM_BASE) {
+    if (gen::HOT_HBM_BASE - c >= RADM2_WARM) WR_stream(v.ws, i, x);
+    else WR(v.ws, i, x);
+    return;
+  }
+  v.regs[-2 - c] = x;
 }
 template <int ARR, int S, int R, int P, int L, class WS>
 __device__ __forceinline__ void launder(HotView<ARR, S, R, P, L, WS>& v) {

@@ -59,6 +59,19 @@ __device__ __forceinline__ void WR(const WsViewT<CPW>& v, int i, double x) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), v.rsrc, v.lane_off,
                                         (v.region + i) * (CPW * 8), 0);
 }
+// The same accesses with the non-temporal cache hint (gfx950's NT bit, bit 1
+// of the buffer instructions' cache-policy operand): the line is streamed
+// instead of kept, so that it does not push warmer lines out of the L2.
+template <int CPW>
+__device__ __forceinline__ double RD_stream(const WsViewT<CPW>& v, int i) {
+  return __builtin_bit_cast(
+      double, __builtin_amdgcn_raw_buffer_load_b64(v.rsrc, v.lane_off, (v.region + i) * (CPW * 8), 2));
+}
+template <int CPW>
+__device__ __forceinline__ void WR_stream(const WsViewT<CPW>& v, int i, double x) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), v.rsrc, v.lane_off,
+                                        (v.region + i) * (CPW * 8), 2);
+}
 // New descriptor from an opaquely "modified" base: loads after this point
 // cannot be merged with loads before it.
 template <int CPW>

@@ -40,14 +40,17 @@ def test_hot_tables_partition_the_ranking(header, budget):
     sizes = (len(G.rate_forms(MECH)[0]), MECH["nvar"], MECH["lu"]["nnz"])
     assert len(ranked) == sum(sizes) and len(set(ranked)) == len(ranked)
     codes = [_array(header, "%s_%s" % (G.hot_name(*budget), n)) for n in ("K", "Y", "G")]
-    lds, regs, split = [], [], []
+    lds, regs, split, hbm = [], [], [], []
     for a, arr in enumerate(codes):
         for i, c in enumerate(arr):
             if c >= 0:
                 lds.append((c, (a, i)))
             elif c <= G.SPLIT_BASE:
                 split.append((G.SPLIT_BASE - c, (a, i)))
-            elif c <= -2:
+            elif c <= G.HBM_BASE:
+                hbm.append((G.HBM_BASE - c, (a, i)))
+            else:
+                assert c <= -2
                 regs.append((-2 - c, (a, i)))
     # LDS slots 0..S-1, registers 0..R-1 and split entries 0..P*L-1 each hold
     # exactly one entry ...
@@ -60,8 +63,10 @@ def test_hot_tables_partition_the_ranking(header, budget):
     assert {e for _, e in lds} == set(ranked[:S])
     assert {e for _, e in regs} == set(ranked[S:S + R])
     assert {e for _, e in split} == set(ranked[S + R:S + R + P * L])
+    # the rest in the workspace, numbered in ranking order
+    n = S + R + nsplit
+    assert [e for _, e in sorted(hbm)] == ranked[n:]
     # and nothing outside the tiers is accessed more than what is inside
-    n = S + R + P * L
     inside = min(acc[e] for e in ranked[:n]) if n else 0
     assert all(acc[e] <= inside for e in ranked[n:])
 
