@@ -77,6 +77,8 @@ def parse():
     ap.add_argument("--paths", default="grid,persistent,learned,surrogate")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the end-to-end, family, batched and accuracy legs (profiling runs)")
+    ap.add_argument("--extras", default="attainable,end_to_end,family,batched,accuracy",
+                    help="which of the extra legs run (comma-separated)")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the learned leg's constant-h0 ceiling sweep")
     ap.add_argument("--cpu-sample", type=int, default=65536)
@@ -203,6 +205,8 @@ def main():
     ends["ref"] = acc_ref
     _same_cells(ends, 0, cells["conc"])
     names = args.paths.split(",")
+    if "grid" not in names and "persistent" not in names:
+        names.append("persistent")      # `value` is the persistent path's rate
     for name in ("grid", "persistent"):
         if name not in names:
             continue
@@ -259,7 +263,8 @@ def main():
         out["learned"] = learned_leg(solver, args, torch, ends, na)
     if "surrogate" in names:
         out["surrogate"] = surrogate_leg(solver, args, torch, ends)
-    if not args.no_extra:
+    extras = set() if args.no_extra else set(args.extras.split(","))
+    if "attainable" in extras:
         # measured attainable peaks beside the published ones
         tf = solver.measure_fp64_peak()
         copy_gbs, read_gbs = attainable_memory(torch)
@@ -271,11 +276,14 @@ def main():
         rl["frac_of_attainable"] = achieved / tf
         _log("attainable: FP64 FMA %.1f TFLOP/s, copy %.0f GB/s, read %.0f GB/s"
              % (tf, copy_gbs, read_gbs))
+    if "end_to_end" in extras:
         out["end_to_end"] = end_to_end(solver, cells, args, torch)
+    if "family" in extras:
         out["family"] = family(solver, batch, conc0, args, torch)
+    if "batched" in extras:
         out["batched"] = batched(solver, args, torch)
-        if acc_ref is not None:
-            out["accuracy"] = accuracy(solver, batch, conc0, ends, acc_ref, args, torch)
+    if "accuracy" in extras and acc_ref is not None:
+        out["accuracy"] = accuracy(solver, batch, conc0, ends, acc_ref, args, torch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cells, args)
     print(json.dumps(out), flush=True)
