@@ -199,6 +199,29 @@ __device__ __forceinline__ void stage_fun(KA k, VA v, double h2o, double m, FA f
   else gen::fun<RF>(k, v, h2o, m, f);
 }
 
+// The stage-2 state y + K1 read where the tendency uses it: entry i is
+// RD(y, i) + RD(k1, i), the addition the stored stage state was formed
+// with, so the state is neither written out nor read back.  In the
+// load-scheduled code both operands are loaded with the batch and added at
+// the use.
+template <class YA, class KA>
+struct SumView {
+  YA y;
+  KA k;
+};
+struct SumRead {
+  double a, b;
+};
+template <int I, class YA, class KA>
+__device__ __forceinline__ SumRead RDP(const SumView<YA, KA>& v) {
+  return SumRead{V(RDP<I>(v.y)), RD(v.k, I)};
+}
+__device__ __forceinline__ double V(const SumRead& r) { return r.a + r.b; }
+template <class YA, class KA>
+__device__ __forceinline__ double RD(const SumView<YA, KA>& v, int i) {
+  return RD(v.y, i) + RD(v.k, i);
+}
+
 // Element-wise pass out[i] = fn(i, {in0[i], in1[i], ..}) over the 59
 // entries, CH at a time: all inputs of a chunk are loaded before any of its
 // outputs is stored, and the next chunk's inputs before this chunk's stores
@@ -478,11 +501,17 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
   double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
   const double rtol = o.rtol, atol = o.atol;
   double err = 0.0;
+  // the fused stages' right-hand side, solved in place (registers)
+  double x[NVAR];
   if constexpr (is_ws_like<GA>::value) {
     // Fused stages: each right-hand side is formed in registers and solved
     // there; K1 and K2 are stored once, K3 never (the combination uses it
-    // in place), and F2 once for the third stage.  The decomposition does
-    // not depend on F0, so it comes first.
+    // in place), and F2 once for the third stage.  The stage-2
+    // state y + K1 is formed where the tendency reads it (never stored),
+    // the third right-hand side takes K2 from the registers the second
+    // solve leaves it in, and on the Ros3 kernels Ynew stays in registers
+    // until the step is accepted.  The decomposition does not depend on
+    // F0, so it comes first.
     // G = -J(y) + ghinv*I formed row by row inside its own LU decomposition
     if constexpr (FAMILY) {
       // the Jacobian at T; ros_PrepareMatrix (KPP): a singular matrix halves
@@ -528,7 +557,6 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
         }
       }
     } else {
-    double x[NVAR];
     // forced: dF/dT in the seventh stage region, the stage times and gamma
     // sums of Ros3, each term in the general stage loop's order
     VA dfdt = k1;
@@ -569,11 +597,10 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
     for (int i = 0; i < NVAR; i++) WR(k1, i, x[i]);
     // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
     //   (forced: F2 at T + alpha_2 H, + H gamma_2 dF/dT)
-    vmap<16>(k2, [](int, const double* a) { return a[0] + a[1]; }, y, k1);
     if constexpr (FAMILY) {
       if (fused_forced) photo_at(k, fc, s.T + r3.alpha[1] * H);
     }
-    gen::fun_pf16_reg(k, k2, h2o, m, x);
+    gen::fun_pf16_reg(k, SumView<YA, VA>{y, k1}, h2o, m, x);
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
     xpass<16>([&x, hc21](int i, const double* a) { x[i] = fma(a[0], hc21, x[i]); }, k1);
@@ -586,11 +613,11 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
     gen::solve_pf16_reg(g, x);
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(k2, i, x[i]);
-    // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2)
-    //   (forced: + H gamma_3 dF/dT)
+    // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2), K2 still in
+    // registers   (forced: + H gamma_3 dF/dT)
     xpass<16>([&x, hc31, hc32](int i, const double* a) {
-      x[i] = fma(a[0], hc32, fma(a[1], hc31, a[2]));
-    }, k2, k1, f);
+      x[i] = fma(x[i], hc32, fma(a[0], hc31, a[1]));
+    }, k1, f);
     if constexpr (FAMILY) {
       if (fused_forced) {
         const double hg = H * r3.gamma[2];
@@ -598,7 +625,9 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
       }
     }
     gen::solve_pf16_reg(g, x);
-    // combination and WRMS error norm with K3 in registers; Ynew kept in f
+    // combination and WRMS error norm with K3 in registers; Ynew replaces
+    // K3 in x on the Ros3 kernels (written to y on acceptance), in f on the
+    // family's (whose other methods leave it there)
     xpass<8>([&x, &err, f, rtol, atol](int i, const double* a) {
       const double a1 = a[0], a2 = a[1], a3 = x[i], yi = a[2];
       const double sm = fma(ROS_M3, a3, fma(ROS_M2, a2, ROS_M1 * a1));
@@ -607,7 +636,8 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
       const double ymax = dmax_ref(fabs(yi), fabs(yn));
       const double q = e / fma(rtol, ymax, atol);
       err = fma(q, q, err);
-      WR(f, i, yn);
+      if constexpr (FAMILY) WR(f, i, yn);
+      else x[i] = yn;
     }, k1, k2, y);
     }
   } else {
@@ -647,9 +677,14 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
   double Hnew = H * fac;
   s.ns++;
   if (err <= 1.0 || H <= o.hmin) {
-    launder(f);
-    launder(y);
-    vmap<16>(y, [](int, const double* a) { return a[0]; }, f);
+    if constexpr (!FAMILY && is_ws_like<GA>::value) {
+#pragma unroll
+      for (int i = 0; i < NVAR; i++) WR(y, i, x[i]);
+    } else {
+      launder(f);
+      launder(y);
+      vmap<16>(y, [](int, const double* a) { return a[0]; }, f);
+    }
     s.T = s.T + H;
     if (s.na == 0) s.hfirst = H * fsafe;  // uncapped proposal (learned-h0 target)
     if (s.rejLast) Hnew = dmin_ref(Hnew, H);

@@ -370,7 +370,8 @@ def test_autotuner_picks_a_compiled_configuration(solver, cells):
         assert torch.equal(b.conc, c0)
         finite = {k: v for k, v in r["timings"].items() if np.isfinite(v)}
         assert json.dumps(r["best"], sort_keys=True) in finite
-        assert len(finite) >= 15
+        # every kernel variant, and at least one other schedule
+        assert len(finite) >= len(autotune.KERNELS) + 1
         # every kernel variant and every schedule of the search was timed
         seen = [json.loads(k) for k in finite]
         assert {(c["placement"], c["waves"]) for c in seen} >= set(autotune.KERNELS)
