@@ -508,6 +508,127 @@ def gen_solve_pf(mech, G):
     return "\n".join(out)
 
 
+def recomputed_l(mech, S=160, R=128, P=0, L=1, maxmod=0, maxterms=1 << 30):
+    """L positions (row k, column j < k) that the Ros3 solves re-form
+    instead of reading back, with the decomposition's own operations:
+    L(k, j) = (Jacobian expression - sum over the row's earlier
+    eliminations i that reach column j of L(k, i) U(i, j), in order) times
+    row j's inverse pivot.  Eligible: at most `maxmod` such eliminations,
+    at most `maxterms` Jacobian terms, and the entry's rate constants,
+    concentrations, U(i, j) and inverse
+    pivot all on chip under hot set (S, R, P, L); the L(k, i) are at hand,
+    formed or loaded by the same row's forward substitution just before.
+    Valid while the rate constants and the state do not change between
+    the decomposition and the solves (autonomous Ros3)."""
+    import re
+    lu = mech["lu"]
+    crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
+    terms = jac_terms(mech)
+    _, forms = rate_forms(mech)
+    code = hot_codes(mech, S, R, P, L)
+    pos = lu_positions(mech)
+
+    def onchip(a, i):
+        return code[a][i] > HBM_BASE
+    out = set()
+    for k in range(59):
+        for kk in range(crow[k], diag[k]):
+            j = icol[kk]
+            mods = l_updates(mech, k, j, pos)
+            e = _entry_expr(terms[kk], forms) or "0.0"
+            if len(mods) > maxmod or len(terms[kk]) > maxterms:
+                continue
+            refs = re.findall(r"RD\((k|v), (\d+)\)", e)
+            if (all(onchip(0 if a == "k" else 1, int(i)) for a, i in refs) and onchip(2, diag[j])
+                    and all(onchip(2, pos[(i, j)]) for i in mods)):
+                out.add(kk)
+    return frozenset(out)
+
+
+def lu_positions(mech):
+    """(row, column) -> LU position."""
+    lu = mech["lu"]
+    return {(r, lu["icol"][kk]): kk for r in range(59) for kk in range(lu["crow"][r], lu["crow"][r + 1])}
+
+
+def l_updates(mech, k, j, pos):
+    """The earlier eliminations of row k that update its column j < k: the
+    columns i < j of row k's L part whose pivot row has U(i, j), in the
+    decomposition's order."""
+    lu = mech["lu"]
+    crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
+    return [icol[q] for q in range(crow[k], diag[k])
+            if icol[q] < j and pos.get((icol[q], j), -1) > diag[icol[q]]]
+
+
+def gen_solve_rc_pf_reg(mech, G, rc, name):
+    """solve_pf<G>_reg() for an LU whose positions `rc` were not stored: each
+    such L(k, j) is formed at its use as the decomposition formed it
+    (recomputed_l()); every other entry is read as before.  The same
+    operations in the same order; the LU loads are issued G at a time ahead
+    of their use, the on-chip inputs of the formed entries at their use
+    (schedule_loads)."""
+    import re
+    lu = mech["lu"]
+    crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
+    terms = jac_terms(mech)
+    _, forms = rate_forms(mech)
+    pos = lu_positions(mech)
+    loads, ops = [], []
+
+    def load(nm, src):
+        loads.append((nm, src, None))
+        return len(loads) - 1
+    for i in range(59):
+        lval = {}          # column -> (name of L(i, column), its load index or None)
+        for kk in range(crow[i], diag[i]):
+            j = icol[kk]
+            if kk in rc:
+                used = []
+                mine = {}
+
+                def rd(nm, src, used=used, mine=mine):
+                    if nm not in mine:
+                        mine[nm] = load(nm, src)
+                        used.append(mine[nm])
+                    return nm
+
+                def sub(mo, kk=kk, rd=rd):
+                    return rd("%s%s_e%d" % (mo.group(1), mo.group(2), kk),
+                              "RD(%s, %s)" % (mo.group(1), mo.group(2)))
+                # a fill-in position starts from 0.0, as in the decomposition
+                e = re.sub(r"RD\((k|v), (\d+)\)", sub, _entry_expr(terms[kk], forms) or "0.0")
+                for c in l_updates(mech, i, j, pos):
+                    ln, li = lval[c]
+                    if li is not None:
+                        used.append(li)
+                    u = rd("u%d_e%d" % (pos[(c, j)], kk), "RD(g, %d)" % pos[(c, j)])
+                    e = "fma(-%s, %s, %s)" % (ln, u, e)
+                p = rd("p%d_e%d" % (diag[j], kk), "RD(g, %d)" % diag[j])
+                ops.append(("const double l%d = %s * %s; x[%d] = fma(-l%d, x[%d], x[%d]);"
+                            % (kk, e, p, i, kk, j, i), used, None))
+                lval[j] = ("l%d" % kk, None)
+            else:
+                li = load("lu%d" % kk, "RD(g, %d)" % kk)
+                ops.append(("x[%d] = fma(-lu%d, x[%d], x[%d]);" % (i, kk, j, i), [li], None))
+                lval[j] = ("lu%d" % kk, li)
+    for i in range(58, -1, -1):
+        for kk in range(diag[i] + 1, crow[i + 1]):
+            ops.append(("x[%d] = fma(-lu%d, x[%d], x[%d]);" % (i, kk, icol[kk], i),
+                        [load("lu%d" % kk, "RD(g, %d)" % kk)], None))
+        ops.append(("x[%d] = x[%d] * lu%d;" % (i, i, diag[i]),
+                    [load("lu%d" % diag[i], "RD(g, %d)" % diag[i])], None))
+    body, _ = schedule_loads(ops, loads, G, lambda nm: nm.startswith("lu"))
+    out = ["// solve_pf%d_reg() with %d L entries formed at their use (recomputed_l())"
+           % (G, len(rc)),
+           "template <class KA, class VA, class GA>",
+           "__device__ __forceinline__ void %s(KA k, VA v, double h2o, double m, GA g,"
+           " double (&x)[59]) {" % name]
+    out += body
+    out.append("}")
+    return "\n".join(out)
+
+
 def deferred_src(src):
     """RD(a, i) -> RDP<i>(a): a read issued with its batch, except for an
     entry in a register tier split over a cell's lanes, whose cross-lane
@@ -527,24 +648,34 @@ def wrap_uses(stmt, names):
     return stmt
 
 
-def schedule_loads(ops, loads, G):
+def schedule_loads(ops, loads, G, batched_if=None):
     """Body lines for `ops` (statement, [load indices], stored entry or None)
     with the `loads` (name, source, producing entry or None) issued in
     batches of G behind scheduling fences, so that between G and 2G loads
     run ahead of their first use; a load waits for the op storing its
-    entry.  Returns (lines, number of empty batches)."""
+    entry.  batched_if: a predicate on a load's name; the loads it rejects
+    are emitted right before the first op that uses them, inside that op's
+    scheduling region.  Returns (lines, number of empty batches)."""
     ops = [(wrap_uses(st, [loads[u][0] for u in used]), used, sto) for st, used, sto in ops]
     store_op = {st: i for i, (_, _, st) in enumerate(ops) if st is not None}
     fence = "  __builtin_amdgcn_sched_barrier(0);"
+    batched = [u for u in range(len(loads)) if batched_if is None or batched_if(loads[u][0])]
+    pos = {u: n for n, u in enumerate(batched)}
     out = []
     issued = 0
     stalls = 0
+    defined = set()
+    frontier = -1
     for i, (st, used, _) in enumerate(ops):
-        nm = max(used) if used else -1
-        while issued <= nm or (issued - nm - 1 < G and issued < len(loads)):
+        mine = [pos[u] for u in used if u in pos]
+        if batched_if is None:
+            nm = max(mine) if mine else -1
+        else:
+            nm = frontier = max(mine + [frontier])
+        while issued <= nm or (issued - nm - 1 < G and issued < len(batched)):
             batch = []
-            while len(batch) < G and issued < len(loads):
-                name, src, dep = loads[issued]
+            while len(batch) < G and issued < len(batched):
+                name, src, dep = loads[batched[issued]]
                 if dep is not None and store_op[dep] >= i:
                     break            # its producer is not emitted yet
                 batch.append("  const auto %s = %s;" % (name, deferred_src(src)))
@@ -556,8 +687,12 @@ def schedule_loads(ops, loads, G):
             out.append(fence)
             out += batch
             out.append(fence)
+        for u in used:
+            if u not in pos and u not in defined:
+                defined.add(u)
+                out.append("  const auto %s = %s;" % (loads[u][0], deferred_src(loads[u][1])))
         out.append("  " + st)
-    assert issued == len(loads)
+    assert issued == len(batched)
     return out, stalls
 
 
@@ -658,14 +793,15 @@ def gen_fun_pf_reg(mech, G, window=16):
     return "\n".join(out)
 
 
-def gen_jac_decomp_pf(mech, G):
+def gen_jac_decomp_pf(mech, G, skip=frozenset(), name=None):
     """jac_decomp() with every load scheduled explicitly: the same
     operations in the same order, but rate constants, concentrations and
     earlier LU rows are loaded in batches of G, issued so that between G and
     2G loads run ahead of their first use (scheduling fences keep the
     compiler from sinking them).  Within a row each rate constant and
     concentration is loaded once.  A load of an LU entry is never issued
-    before the store that produces it."""
+    before the store that produces it.  skip: LU positions not stored (the
+    solves recompute them, recomputed_l())."""
     import re
     lu = mech["lu"]
     crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
@@ -707,13 +843,15 @@ def gen_jac_decomp_pf(mech, G):
                 ops.append(("w%d_%d = fma(-w%d_%d, u%d_%d, w%d_%d);" % (c, k, j, k, jj, k, c, k),
                             [li], None))
         for kk in range(crow[k], crow[k + 1]):
+            if kk in skip:
+                continue
             c = icol[kk]
             v = ("1.0 / w%d_%d" % (c, k)) if kk == diag[k] else ("w%d_%d" % (c, k))
             ops.append(("WR(g, %d, %s);" % (kk, v), [], kk))
     out = ["// jac_decomp() with loads issued %d at a time ahead of their use" % G]
     out.append("template <class KA, class VA, class GA>")
-    out.append("__device__ __forceinline__ int jac_decomp_pf%d(KA k, VA v, double h2o, double m,"
-               " double ghinv, GA g) {" % G)
+    out.append("__device__ __forceinline__ int %s(KA k, VA v, double h2o, double m,"
+               " double ghinv, GA g) {" % (name or "jac_decomp_pf%d" % G))
     out.append("  int sing = 0;")
     body, stalls = schedule_loads(ops, loads, G)
     out += body
@@ -845,11 +983,21 @@ def gen_wrf_tables(mech):
     ])
 
 
+# L entries the Ros3 solves form instead of reading back (recomputed_l()):
+# none that earlier eliminations of its row update, at most two Jacobian
+# terms.  Measured at 1M cells (ms): updates allowed 0 / 1 / 2 / 8: 627.7 /
+# 647 / 655 / 660 (all terms); terms at most 1 / 2 / any: 623.5 / 621.8 /
+# 627.7 (no updates); none formed: 647.
+RC_MAXMOD = 0
+RC_MAXTERMS = 2
+
+
 def generate(dst=None, mech_path=None):
     mech = load_mech(mech_path)
     dst = dst or os.path.join(PKG, "csrc", "generated", "radm2_gen.h")
     os.makedirs(os.path.dirname(dst), exist_ok=True)
     rc, nshared = gen_rconst(mech)
+    rc_l = recomputed_l(mech, maxmod=RC_MAXMOD, maxterms=RC_MAXTERMS)
     dec, nfma = gen_decomp(mech, False)
     jdec, _ = gen_decomp(mech, True)
     parts = [
@@ -913,6 +1061,12 @@ def generate(dst=None, mech_path=None):
         gen_solve_pf_reg(mech, 16),
         "",
         gen_fun_pf_reg(mech, 16),
+        "",
+        "// L entries the Ros3 solves form at their use instead of reading back",
+        "constexpr int RECOMPUTED_L = %d;" % len(rc_l),
+        gen_jac_decomp_pf(mech, 16, rc_l, "jac_decomp_rc_pf16")[0],
+        "",
+        gen_solve_rc_pf_reg(mech, 16, rc_l, "solve_rc_pf16_reg"),
         "",
         gen_hot_tables(mech),
         "",

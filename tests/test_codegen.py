@@ -71,9 +71,16 @@ def test_hot_tables_partition_the_ranking(header, budget):
     assert all(acc[e] <= inside for e in ranked[n:])
 
 
-def test_lu_entries_are_read_after_they_are_stored(header):
-    start = header.index("int jac_decomp_pf16(")
-    body = header[start:header.index("\n}\n", start)]
+def _body(header, sig):
+    start = header.index(sig)
+    return header[start:header.index("\n}\n", start)]
+
+
+@pytest.mark.parametrize("fn", ["jac_decomp_pf16", "jac_decomp_rc_pf16"])
+def test_lu_entries_are_read_after_they_are_stored(header, fn):
+    body = _body(header, "int %s(" % fn)
+    skipped = G.recomputed_l(MECH, maxmod=G.RC_MAXMOD, maxterms=G.RC_MAXTERMS) \
+        if fn == "jac_decomp_rc_pf16" else frozenset()
     stored = {}
     assert len(re.findall(r"RDP<\d+>\(g\)", body)) > 300   # the check is not vacuous
     for n, ln in enumerate(body.split("\n")):
@@ -85,7 +92,45 @@ def test_lu_entries_are_read_after_they_are_stored(header):
             q = int(mo.group(1))
             assert q not in stored, "LU entry %d stored twice" % q
             stored[q] = n
-    assert sorted(stored) == list(range(MECH["lu"]["nnz"]))
+    assert sorted(stored) == sorted(set(range(MECH["lu"]["nnz"])) - skipped)
+
+
+def test_recomputed_l_entries_are_formed_as_the_decomposition_forms_them(header):
+    # the Ros3 solves form these L entries instead of reading them back: each
+    # must be an L position that no earlier elimination of its row updates,
+    # never stored by the decomposition the Ros3 kernels run, never read by
+    # their solve, and formed there by the decomposition's own expression
+    # times the same inverse pivot
+    lu = MECH["lu"]
+    crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
+    rc = G.recomputed_l(MECH, maxmod=G.RC_MAXMOD, maxterms=G.RC_MAXTERMS)
+    assert len(rc) == int(re.search(r"RECOMPUTED_L = (\d+);", header).group(1)) > 50
+    row_of = {kk: k for k in range(59) for kk in range(crow[k], crow[k + 1])}
+    dec = _body(header, "int jac_decomp_pf16(")
+    solve = _body(header, "void solve_rc_pf16_reg(")
+    read = {int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", solve)}
+    assert not (read & rc)
+    # every other LU entry is read, the L part once per forward substitution
+    assert read == set(range(lu["nnz"])) - rc
+
+    def norm(e):
+        return re.sub(r"V\(([kv])(\d+)_(?:e)?\d+\)", r"\1[\2]", e).strip()
+    for kk in sorted(rc):
+        k, j = row_of[kk], icol[kk]
+        assert crow[k] <= kk < diag[k]
+        assert not G.l_updates(MECH, k, j, G.lu_positions(MECH))
+        mo = re.search(r"const double l%d = (.*) \* V\(p(\d+)_e%d\); "
+                       r"x\[%d\] = fma\(-l%d, x\[%d\], x\[%d\]\);"
+                       % (kk, kk, k, kk, j, k), solve)
+        assert mo, kk
+        assert int(mo.group(2)) == diag[j]
+        md = re.search(r"double w%d_%d = (.*);" % (j, k), dec)
+        assert md, kk
+        assert norm(mo.group(1)) == norm(md.group(1)), kk
+        # the decomposition scales the entry by the same inverse pivot and
+        # applies no other update to it before
+        upd = re.findall(r"\n\s*w%d_%d = ([^\n]*);" % (j, k), dec)
+        assert upd[0] == "w%d_%d * V(u%d_%d)" % (j, k, diag[j], k), (kk, upd)
 
 
 def test_workspace_and_hot_accesses_use_constant_indices(header):
