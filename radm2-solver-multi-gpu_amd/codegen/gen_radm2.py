@@ -509,17 +509,18 @@ def gen_solve_pf(mech, G):
 
 
 def recomputed_l(mech, S=160, R=128, P=0, L=1, maxmod=0, maxterms=1 << 30):
-    """L positions (row k, column j < k) that the Ros3 solves re-form
-    instead of reading back, with the decomposition's own operations:
-    L(k, j) = (Jacobian expression - sum over the row's earlier
-    eliminations i that reach column j of L(k, i) U(i, j), in order) times
-    row j's inverse pivot.  Eligible: at most `maxmod` such eliminations,
-    at most `maxterms` Jacobian terms, and the entry's rate constants,
-    concentrations, U(i, j) and inverse
-    pivot all on chip under hot set (S, R, P, L); the L(k, i) are at hand,
-    formed or loaded by the same row's forward substitution just before.
-    Valid while the rate constants and the state do not change between
-    the decomposition and the solves (autonomous Ros3)."""
+    """L positions (row k, column j < k) that the solves re-form instead of
+    reading back, with the decomposition's own operations: L(k, j) =
+    (Jacobian expression - sum over the row's earlier eliminations i that
+    reach column j of L(k, i) U(i, j), in order) times row j's inverse
+    pivot.  Eligible: at most `maxmod` such eliminations, at most
+    `maxterms` Jacobian terms, the entry's rate constants, concentrations,
+    U(i, j) and inverse pivot all on chip under hot set (S, R, P, L), and
+    no photolysis rate constant among them: a non-autonomous integration
+    moves those between the decomposition and the later stages' solves,
+    while the other rate constants and the state stay fixed through an
+    attempt.  The L(k, i) are at hand, formed or loaded by the same row's
+    forward substitution just before."""
     import re
     lu = mech["lu"]
     crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
@@ -527,6 +528,7 @@ def recomputed_l(mech, S=160, R=128, P=0, L=1, maxmod=0, maxterms=1 << 30):
     _, forms = rate_forms(mech)
     code = hot_codes(mech, S, R, P, L)
     pos = lu_positions(mech)
+    nph = len(photo_j(mech))
 
     def onchip(a, i):
         return code[a][i] > HBM_BASE
@@ -539,6 +541,8 @@ def recomputed_l(mech, S=160, R=128, P=0, L=1, maxmod=0, maxterms=1 << 30):
             if len(mods) > maxmod or len(terms[kk]) > maxterms:
                 continue
             refs = re.findall(r"RD\((k|v), (\d+)\)", e)
+            if any(a == "k" and int(i) < nph for a, i in refs):
+                continue
             if (all(onchip(0 if a == "k" else 1, int(i)) for a, i in refs) and onchip(2, diag[j])
                     and all(onchip(2, pos[(i, j)]) for i in mods)):
                 out.add(kk)
@@ -983,7 +987,7 @@ def gen_wrf_tables(mech):
     ])
 
 
-# L entries the Ros3 solves form instead of reading back (recomputed_l()):
+# L entries the solves form instead of reading back (recomputed_l()):
 # none that earlier eliminations of its row update, at most two Jacobian
 # terms.  Measured at 1M cells (ms): updates allowed 0 / 1 / 2 / 8: 627.7 /
 # 647 / 655 / 660 (all terms); terms at most 1 / 2 / any: 623.5 / 621.8 /
