@@ -549,6 +549,46 @@ def recomputed_l(mech, S=160, R=128, P=0, L=1, maxmod=0, maxterms=1 << 30):
     return frozenset(out)
 
 
+def recomputed_u(mech, S=160, R=128, P=0, L=1, maxterms=1 << 30):
+    """Off-diagonal U positions (row k, column c > k) left in the workspace
+    under hot set (S, R, P, L) that no elimination of their row updates:
+    U(k, c) is then the Jacobian expression itself.  Eligible when that
+    expression has at most `maxterms` terms, reads only on-chip rate
+    constants and concentrations and no photolysis rate constant (which the
+    forcing moves within an attempt): the decomposition does not store
+    these, and the later rows that eliminate with row k and the backward
+    substitutions form them at their use."""
+    import re
+    lu = mech["lu"]
+    crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
+    terms = jac_terms(mech)
+    _, forms = rate_forms(mech)
+    code = hot_codes(mech, S, R, P, L)
+    pos = lu_positions(mech)
+    nph = len(photo_j(mech))
+
+    def onchip(a, i):
+        return code[a][i] > HBM_BASE
+    out = set()
+    for k in range(59):
+        lcols = [icol[q] for q in range(crow[k], diag[k])]
+        for kk in range(diag[k] + 1, crow[k + 1]):
+            c = icol[kk]
+            if onchip(2, kk) or len(terms[kk]) > maxterms:
+                continue
+            if any(pos.get((i, c), -1) > diag[i] for i in lcols):
+                continue
+            e = _entry_expr(terms[kk], forms)
+            if e is None:
+                continue
+            refs = re.findall(r"RD\((k|v), (\d+)\)", e)
+            if any(a == "k" and int(i) < nph for a, i in refs):
+                continue
+            if all(onchip(0 if a == "k" else 1, int(i)) for a, i in refs):
+                out.add(kk)
+    return frozenset(out)
+
+
 def lu_positions(mech):
     """(row, column) -> LU position."""
     lu = mech["lu"]
@@ -565,7 +605,7 @@ def l_updates(mech, k, j, pos):
             if icol[q] < j and pos.get((icol[q], j), -1) > diag[icol[q]]]
 
 
-def gen_solve_rc_pf_reg(mech, G, rc, name):
+def gen_solve_rc_pf_reg(mech, G, rc, name, formed_u=frozenset()):
     """solve_pf<G>_reg() for an LU whose positions `rc` were not stored: each
     such L(k, j) is formed at its use as the decomposition formed it
     (recomputed_l()); every other entry is read as before.  The same
@@ -618,6 +658,19 @@ def gen_solve_rc_pf_reg(mech, G, rc, name):
                 lval[j] = ("lu%d" % kk, li)
     for i in range(58, -1, -1):
         for kk in range(diag[i] + 1, crow[i + 1]):
+            if kk in formed_u:
+                used = []
+                mine = {}
+
+                def fsub(mo, used=used, mine=mine, kk=kk):
+                    nm = "f%s%s_u%d" % (mo.group(1), mo.group(2), kk)
+                    if nm not in mine:
+                        mine[nm] = load(nm, "RD(%s, %s)" % (mo.group(1), mo.group(2)))
+                        used.append(mine[nm])
+                    return nm
+                eu = re.sub(r"RD\((k|v), (\d+)\)", fsub, _entry_expr(terms[kk], forms))
+                ops.append(("x[%d] = fma(-(%s), x[%d], x[%d]);" % (i, eu, icol[kk], i), used, None))
+                continue
             ops.append(("x[%d] = fma(-lu%d, x[%d], x[%d]);" % (i, kk, icol[kk], i),
                         [load("lu%d" % kk, "RD(g, %d)" % kk)], None))
         ops.append(("x[%d] = x[%d] * lu%d;" % (i, i, diag[i]),
@@ -652,14 +705,15 @@ def wrap_uses(stmt, names):
     return stmt
 
 
-def schedule_loads(ops, loads, G, batched_if=None):
+def schedule_loads(ops, loads, G, batched_if=None, frontier_rule=True):
     """Body lines for `ops` (statement, [load indices], stored entry or None)
     with the `loads` (name, source, producing entry or None) issued in
     batches of G behind scheduling fences, so that between G and 2G loads
     run ahead of their first use; a load waits for the op storing its
     entry.  batched_if: a predicate on a load's name; the loads it rejects
     are emitted right before the first op that uses them, inside that op's
-    scheduling region.  Returns (lines, number of empty batches)."""
+    scheduling region; with frontier_rule an op without batched loads also
+    tops the look-ahead up.  Returns (lines, number of empty batches)."""
     ops = [(wrap_uses(st, [loads[u][0] for u in used]), used, sto) for st, used, sto in ops]
     store_op = {st: i for i, (_, _, st) in enumerate(ops) if st is not None}
     fence = "  __builtin_amdgcn_sched_barrier(0);"
@@ -672,7 +726,7 @@ def schedule_loads(ops, loads, G, batched_if=None):
     frontier = -1
     for i, (st, used, _) in enumerate(ops):
         mine = [pos[u] for u in used if u in pos]
-        if batched_if is None:
+        if batched_if is None or not frontier_rule:
             nm = max(mine) if mine else -1
         else:
             nm = frontier = max(mine + [frontier])
@@ -797,7 +851,7 @@ def gen_fun_pf_reg(mech, G, window=16):
     return "\n".join(out)
 
 
-def gen_jac_decomp_pf(mech, G, skip=frozenset(), name=None):
+def gen_jac_decomp_pf(mech, G, skip=frozenset(), name=None, formed_u=frozenset()):
     """jac_decomp() with every load scheduled explicitly: the same
     operations in the same order, but rate constants, concentrations and
     earlier LU rows are loaded in batches of G, issued so that between G and
@@ -805,7 +859,9 @@ def gen_jac_decomp_pf(mech, G, skip=frozenset(), name=None):
     compiler from sinking them).  Within a row each rate constant and
     concentration is loaded once.  A load of an LU entry is never issued
     before the store that produces it.  skip: LU positions not stored (the
-    solves recompute them, recomputed_l())."""
+    solves form them, recomputed_l(), recomputed_u()); formed_u: U
+    positions whose reads by later rows are formed from their Jacobian
+    expression (recomputed_u()), their inputs read at the use."""
     import re
     lu = mech["lu"]
     crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
@@ -843,6 +899,17 @@ def gen_jac_decomp_pf(mech, G, skip=frozenset(), name=None):
             ops.append(("w%d_%d = w%d_%d * u%d_%d;" % (j, k, j, k, diag[j], k), [li], None))
             for jj in range(diag[j] + 1, crow[j + 1]):
                 c = icol[jj]
+                if jj in formed_u:
+                    used = []
+
+                    def fsub(mo, used=used, jj=jj, k=k):
+                        nm = "f%s%s_u%d_%d" % (mo.group(1), mo.group(2), jj, k)
+                        used.append(need(nm, "RD(%s, %s)" % (mo.group(1), mo.group(2)), None))
+                        return nm
+                    eu = re.sub(r"RD\((k|v), (\d+)\)", fsub, _entry_expr(terms[jj], forms))
+                    ops.append(("w%d_%d = fma(-w%d_%d, %s, w%d_%d);" % (c, k, j, k, eu, c, k),
+                                used, None))
+                    continue
                 li = need("u%d_%d" % (jj, k), "RD(g, %d)" % jj, jj)
                 ops.append(("w%d_%d = fma(-w%d_%d, u%d_%d, w%d_%d);" % (c, k, j, k, jj, k, c, k),
                             [li], None))
@@ -857,7 +924,9 @@ def gen_jac_decomp_pf(mech, G, skip=frozenset(), name=None):
     out.append("__device__ __forceinline__ int %s(KA k, VA v, double h2o, double m,"
                " double ghinv, GA g) {" % (name or "jac_decomp_pf%d" % G))
     out.append("  int sing = 0;")
-    body, stalls = schedule_loads(ops, loads, G)
+    body, stalls = schedule_loads(ops, loads, G,
+                                  (lambda nm: not nm.startswith("f")) if formed_u else None,
+                                  frontier_rule=False)
     out += body
     out.append("  return sing;")
     out.append("}")
@@ -1002,6 +1071,7 @@ def generate(dst=None, mech_path=None):
     os.makedirs(os.path.dirname(dst), exist_ok=True)
     rc, nshared = gen_rconst(mech)
     rc_l = recomputed_l(mech, maxmod=RC_MAXMOD, maxterms=RC_MAXTERMS)
+    rc_u = recomputed_u(mech, maxterms=RC_MAXTERMS)
     dec, nfma = gen_decomp(mech, False)
     jdec, _ = gen_decomp(mech, True)
     parts = [
@@ -1066,11 +1136,17 @@ def generate(dst=None, mech_path=None):
         "",
         gen_fun_pf_reg(mech, 16),
         "",
-        "// L entries the Ros3 solves form at their use instead of reading back",
+        "// L entries (and, in the _rcu pair, U entries) formed at their use",
+        "// instead of stored and read back",
         "constexpr int RECOMPUTED_L = %d;" % len(rc_l),
+        "constexpr int RECOMPUTED_U = %d;" % len(rc_u),
         gen_jac_decomp_pf(mech, 16, rc_l, "jac_decomp_rc_pf16")[0],
         "",
         gen_solve_rc_pf_reg(mech, 16, rc_l, "solve_rc_pf16_reg"),
+        "",
+        gen_jac_decomp_pf(mech, 16, rc_l | rc_u, "jac_decomp_rcu_pf16", rc_u)[0],
+        "",
+        gen_solve_rc_pf_reg(mech, 16, rc_l, "solve_rcu_pf16_reg", rc_u),
         "",
         gen_hot_tables(mech),
         "",

@@ -76,11 +76,17 @@ def _body(header, sig):
     return header[start:header.index("\n}\n", start)]
 
 
-@pytest.mark.parametrize("fn", ["jac_decomp_pf16", "jac_decomp_rc_pf16"])
+def _formed():
+    return (G.recomputed_l(MECH, maxmod=G.RC_MAXMOD, maxterms=G.RC_MAXTERMS),
+            G.recomputed_u(MECH, maxterms=G.RC_MAXTERMS))
+
+
+@pytest.mark.parametrize("fn", ["jac_decomp_pf16", "jac_decomp_rc_pf16", "jac_decomp_rcu_pf16"])
 def test_lu_entries_are_read_after_they_are_stored(header, fn):
     body = _body(header, "int %s(" % fn)
-    skipped = G.recomputed_l(MECH, maxmod=G.RC_MAXMOD, maxterms=G.RC_MAXTERMS) \
-        if fn == "jac_decomp_rc_pf16" else frozenset()
+    rc_l, rc_u = _formed()
+    skipped = {"jac_decomp_pf16": frozenset(), "jac_decomp_rc_pf16": rc_l,
+               "jac_decomp_rcu_pf16": rc_l | rc_u}[fn]
     stored = {}
     assert len(re.findall(r"RDP<\d+>\(g\)", body)) > 300   # the check is not vacuous
     for n, ln in enumerate(body.split("\n")):
@@ -171,3 +177,42 @@ def test_photolysis_map_matches_the_rate_laws():
     for r, s in enumerate(pj):
         assert MECH["rates"][r].strip() == "(J[%d])" % s
     assert sorted(pj) == list(range(21))
+
+
+def test_recomputed_u_entries_are_formed_as_the_decomposition_forms_them(header):
+    # the family kernel's pair also forms U entries: each an off-diagonal U
+    # position that no elimination of its row updates, so that it is the
+    # Jacobian expression itself; the decomposition neither stores nor reads
+    # it (the rows that eliminate with it form it), the solve forms it in the
+    # backward substitution
+    lu = MECH["lu"]
+    crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
+    rc_l, rc_u = _formed()
+    assert len(rc_u) == int(re.search(r"RECOMPUTED_U = (\d+);", header).group(1)) > 20
+    row_of = {kk: k for k in range(59) for kk in range(crow[k], crow[k + 1])}
+    dec0 = _body(header, "int jac_decomp_pf16(")
+    dec = _body(header, "int jac_decomp_rcu_pf16(")
+    solve = _body(header, "void solve_rcu_pf16_reg(")
+    read = {int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", solve)}
+    assert read == set(range(lu["nnz"])) - rc_l - rc_u
+    assert not ({int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", dec)} & rc_u)
+
+    def norm(e):
+        return re.sub(r"V\(f?([kv])(\d+)_[u\d_]+\)", r"\1[\2]", e).strip()
+    pos = G.lu_positions(MECH)
+    for kk in sorted(rc_u):
+        k, c = row_of[kk], icol[kk]
+        assert diag[k] < kk < crow[k + 1]
+        assert not any(pos.get((i, c), -1) > diag[i]
+                       for i in (icol[q] for q in range(crow[k], diag[k])))
+        md = re.search(r"double w%d_%d = (.*);" % (c, k), dec0)
+        assert md, kk
+        assert not re.search(r"\n\s*w%d_%d = " % (c, k), dec0), kk   # never updated
+        expr = norm(md.group(1))
+        mo = re.search(r"x\[%d\] = fma\(-\((.*)\), x\[%d\], x\[%d\]\);" % (k, c, k), solve)
+        assert mo and norm(mo.group(1)) == expr, kk
+        # every later row that eliminates with row k forms it the same way
+        for r in range(k + 1, 59):
+            if (r, k) in pos:
+                mr = re.search(r"w%d_%d = fma\(-w%d_%d, (.*), w%d_%d\);" % (c, r, k, r, c, r), dec)
+                assert mr and norm(mr.group(1)) == expr, (kk, r)
