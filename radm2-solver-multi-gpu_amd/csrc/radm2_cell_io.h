@@ -5,15 +5,6 @@
 
 namespace radm2 {
 
-// The cell arrays are global memory.  Their pointers come from the domain
-// table, so without a cast the compiler would access them as generic (flat)
-// addresses, which it must keep in order with every LDS access, and whose
-// loads wait on both memory counters.
-template <class T>
-__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
-  return (__attribute__((address_space(1))) T*)p;
-}
-
 // Domain of global work index g: last domain with first <= g.
 __device__ __forceinline__ int find_domain(const DomainDesc* __restrict__ doms, int ndom,
                                            int64_t g) {

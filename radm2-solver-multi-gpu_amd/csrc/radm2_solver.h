@@ -331,13 +331,15 @@ struct Forcing {
   double dt;
 };
 
-// The photolysis rate constants (a bare J each, mgpu.cu:763-783) at time t.
+// The photolysis rate constants (a bare J each, mgpu.cu:763-783) at time t,
+// J read as global memory.
 template <class KA>
 __device__ __forceinline__ void photo_at(KA k, const Forcing& fc, double t) {
   const double w = t / fc.dt;
+  const auto j0 = gmem(fc.j0), j1 = gmem(fc.j1);
 #pragma unroll
   for (int r = 0; r < gen::NPHOTO; r++) {
-    const double a = fc.j0[gen::PHOTO_J[r] * fc.ld], b = fc.j1[gen::PHOTO_J[r] * fc.ld];
+    const double a = j0[gen::PHOTO_J[r] * fc.ld], b = j1[gen::PHOTO_J[r] * fc.ld];
     WR(k, r, fma(w, b - a, a));
   }
 }

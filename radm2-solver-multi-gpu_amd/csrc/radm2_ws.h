@@ -80,6 +80,15 @@ __device__ __forceinline__ void launder(WsViewT<CPW>& v) {
   v.rsrc = __builtin_amdgcn_make_buffer_rsrc(v.wave_base, (short)0, v.nbytes, kBufferWord3);
 }
 
+// A pointer to global memory as a global-address pointer.  The cell arrays'
+// pointers come from the domain table, so without the cast the compiler
+// would access them as generic (flat) addresses, which it must keep in
+// order with every LDS access, and whose loads wait on both memory counters.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ double RD(const double* a, int i) { return a[i]; }
 __device__ __forceinline__ void WR(double* a, int i, double x) { a[i] = x; }
 __device__ __forceinline__ void launder(double*) {}
