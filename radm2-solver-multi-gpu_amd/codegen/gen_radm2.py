@@ -1136,17 +1136,12 @@ def generate(dst=None, mech_path=None):
         "",
         gen_fun_pf_reg(mech, 16),
         "",
-        "// L entries (and, in the _rcu pair, U entries) formed at their use",
-        "// instead of stored and read back",
+        "// L and U entries formed at their use instead of stored and read back",
         "constexpr int RECOMPUTED_L = %d;" % len(rc_l),
         "constexpr int RECOMPUTED_U = %d;" % len(rc_u),
-        gen_jac_decomp_pf(mech, 16, rc_l, "jac_decomp_rc_pf16")[0],
+        gen_jac_decomp_pf(mech, 16, rc_l | rc_u, "jac_decomp_rc_pf16", rc_u)[0],
         "",
-        gen_solve_rc_pf_reg(mech, 16, rc_l, "solve_rc_pf16_reg"),
-        "",
-        gen_jac_decomp_pf(mech, 16, rc_l | rc_u, "jac_decomp_rcu_pf16", rc_u)[0],
-        "",
-        gen_solve_rc_pf_reg(mech, 16, rc_l, "solve_rcu_pf16_reg", rc_u),
+        gen_solve_rc_pf_reg(mech, 16, rc_l, "solve_rc_pf16_reg", rc_u),
         "",
         gen_hot_tables(mech),
         "",

@@ -193,26 +193,22 @@ __device__ __forceinline__ int stage_jac_decomp(KA k, YA y, double h2o, double m
   else return gen::jac_decomp<R>(k, y, h2o, m, ghinv, g);
 }
 
-// The fused stages' decomposition and solves.  The L entries that the
-// decomposition does not store are formed by the solves at their use from
-// the same rate constants, concentrations and inverse pivot, by the same
-// operations: the state and the rate constants they use (no photolysis
-// rate, which the forcing moves between stages) are fixed through an
-// attempt.  The family kernel also forms a set of U entries (the _rcu
-// pair): its longer stage loops read them more often, while in the Ros3
-// kernels the extra registers cost more than the loads they save.
-template <bool FAMILY, class KA, class YA, class GA>
+// The fused stages' decomposition and solves.  The L and U entries that
+// the decomposition does not store are formed at their use (by the solves,
+// and for U also by the rows that eliminate with them) from the same rate
+// constants, concentrations and inverse pivots, by the same operations:
+// the state and the rate constants they use (no photolysis rate, which the
+// forcing moves between stages) are fixed through an attempt.
+template <class KA, class YA, class GA>
 __device__ __forceinline__ int stage_decomp(KA k, YA y, double h2o, double m, double ghinv,
                                             GA g) {
-  if constexpr (FAMILY) return gen::jac_decomp_rcu_pf16(k, y, h2o, m, ghinv, g);
-  else return gen::jac_decomp_rc_pf16(k, y, h2o, m, ghinv, g);
+  return gen::jac_decomp_rc_pf16(k, y, h2o, m, ghinv, g);
 }
 
-template <bool FAMILY, class KA, class YA, class GA>
+template <class KA, class YA, class GA>
 __device__ __forceinline__ void stage_solve_reg(KA k, YA y, double h2o, double m, GA g,
                                                 double (&x)[NVAR]) {
-  if constexpr (FAMILY) gen::solve_rcu_pf16_reg(k, y, h2o, m, g, x);
-  else gen::solve_rc_pf16_reg(k, y, h2o, m, g, x);
+  gen::solve_rc_pf16_reg(k, y, h2o, m, g, x);
 }
 
 template <int RF, class KA, class VA, class FA>
@@ -437,7 +433,7 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
       launder(d);
       xpass<16>([&x, hg](int i, const double* v) { x[i] = fma(hg, v[0], x[i]); }, d);
     }
-    stage_solve_reg<true>(k, y, h2o, m, g, x);
+    stage_solve_reg(k, y, h2o, m, g, x);
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(ks, i, x[i]);
   }
@@ -555,7 +551,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
       // code: none, mgpu.cu:3147-3163), then -8
       if (forced) photo_at(k, fc, s.T);
       for (int nsc = 0;; nsc++) {
-        if (stage_decomp<FAMILY>(k, y, h2o, m, 1.0 / (H * gam), g) == 0) break;
+        if (stage_decomp(k, y, h2o, m, 1.0 / (H * gam), g) == 0) break;
         if (nsc >= o.sing_retries) {
           s.H = H;
           s.status = ST_SINGULAR;
@@ -567,7 +563,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
       hc31 = ROS_C31 / H;
       hc32 = ROS_C32 / H;
     } else {
-      if (stage_decomp<FAMILY>(k, y, h2o, m, ghinv, g) != 0) {
+      if (stage_decomp(k, y, h2o, m, ghinv, g) != 0) {
         s.status = ST_SINGULAR;
         return;
       }
@@ -628,7 +624,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
     } else {
       gen::fun_pf16_reg(k, y, h2o, m, x);
     }
-    stage_solve_reg<FAMILY>(k, y, h2o, m, g, x);
+    stage_solve_reg(k, y, h2o, m, g, x);
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(k1, i, x[i]);
     // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
@@ -646,7 +642,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
         xpass<16>([&x, hg](int i, const double* a) { x[i] = fma(hg, a[0], x[i]); }, dfdt);
       }
     }
-    stage_solve_reg<FAMILY>(k, y, h2o, m, g, x);
+    stage_solve_reg(k, y, h2o, m, g, x);
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(k2, i, x[i]);
     // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2), K2 still in
@@ -660,7 +656,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
         xpass<16>([&x, hg](int i, const double* a) { x[i] = fma(hg, a[0], x[i]); }, dfdt);
       }
     }
-    stage_solve_reg<FAMILY>(k, y, h2o, m, g, x);
+    stage_solve_reg(k, y, h2o, m, g, x);
     // combination and WRMS error norm with K3 in registers; Ynew replaces
     // K3 in x on the Ros3 kernels (written to y on acceptance), in f on the
     // family's (whose other methods leave it there)

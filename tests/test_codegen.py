@@ -81,12 +81,11 @@ def _formed():
             G.recomputed_u(MECH, maxterms=G.RC_MAXTERMS))
 
 
-@pytest.mark.parametrize("fn", ["jac_decomp_pf16", "jac_decomp_rc_pf16", "jac_decomp_rcu_pf16"])
+@pytest.mark.parametrize("fn", ["jac_decomp_pf16", "jac_decomp_rc_pf16"])
 def test_lu_entries_are_read_after_they_are_stored(header, fn):
     body = _body(header, "int %s(" % fn)
     rc_l, rc_u = _formed()
-    skipped = {"jac_decomp_pf16": frozenset(), "jac_decomp_rc_pf16": rc_l,
-               "jac_decomp_rcu_pf16": rc_l | rc_u}[fn]
+    skipped = {"jac_decomp_pf16": frozenset(), "jac_decomp_rc_pf16": rc_l | rc_u}[fn]
     stored = {}
     assert len(re.findall(r"RDP<\d+>\(g\)", body)) > 300   # the check is not vacuous
     for n, ln in enumerate(body.split("\n")):
@@ -102,22 +101,22 @@ def test_lu_entries_are_read_after_they_are_stored(header, fn):
 
 
 def test_recomputed_l_entries_are_formed_as_the_decomposition_forms_them(header):
-    # the Ros3 solves form these L entries instead of reading them back: each
+    # the solves form these L entries instead of reading them back: each
     # must be an L position that no earlier elimination of its row updates,
-    # never stored by the decomposition the Ros3 kernels run, never read by
-    # their solve, and formed there by the decomposition's own expression
-    # times the same inverse pivot
+    # never stored by the decomposition the kernels run, never read by their
+    # solve, and formed there by the decomposition's own expression times
+    # the same inverse pivot
     lu = MECH["lu"]
     crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
-    rc = G.recomputed_l(MECH, maxmod=G.RC_MAXMOD, maxterms=G.RC_MAXTERMS)
+    rc, rc_u = _formed()
     assert len(rc) == int(re.search(r"RECOMPUTED_L = (\d+);", header).group(1)) > 50
     row_of = {kk: k for k in range(59) for kk in range(crow[k], crow[k + 1])}
     dec = _body(header, "int jac_decomp_pf16(")
     solve = _body(header, "void solve_rc_pf16_reg(")
     read = {int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", solve)}
     assert not (read & rc)
-    # every other LU entry is read, the L part once per forward substitution
-    assert read == set(range(lu["nnz"])) - rc
+    # every other LU entry is read (the formed U entries are formed too)
+    assert read == set(range(lu["nnz"])) - rc - rc_u
 
     def norm(e):
         return re.sub(r"V\(([kv])(\d+)_(?:e)?\d+\)", r"\1[\2]", e).strip()
@@ -180,7 +179,7 @@ def test_photolysis_map_matches_the_rate_laws():
 
 
 def test_recomputed_u_entries_are_formed_as_the_decomposition_forms_them(header):
-    # the family kernel's pair also forms U entries: each an off-diagonal U
+    # the decomposition and the solves also form U entries: each an off-diagonal U
     # position that no elimination of its row updates, so that it is the
     # Jacobian expression itself; the decomposition neither stores nor reads
     # it (the rows that eliminate with it form it), the solve forms it in the
@@ -191,8 +190,8 @@ def test_recomputed_u_entries_are_formed_as_the_decomposition_forms_them(header)
     assert len(rc_u) == int(re.search(r"RECOMPUTED_U = (\d+);", header).group(1)) > 20
     row_of = {kk: k for k in range(59) for kk in range(crow[k], crow[k + 1])}
     dec0 = _body(header, "int jac_decomp_pf16(")
-    dec = _body(header, "int jac_decomp_rcu_pf16(")
-    solve = _body(header, "void solve_rcu_pf16_reg(")
+    dec = _body(header, "int jac_decomp_rc_pf16(")
+    solve = _body(header, "void solve_rc_pf16_reg(")
     read = {int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", solve)}
     assert read == set(range(lu["nnz"])) - rc_l - rc_u
     assert not ({int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", dec)} & rc_u)
