@@ -1056,13 +1056,19 @@ def gen_wrf_tables(mech):
     ])
 
 
-# L entries the solves form instead of reading back (recomputed_l()):
-# none that earlier eliminations of its row update, at most two Jacobian
-# terms.  Measured at 1M cells (ms): updates allowed 0 / 1 / 2 / 8: 627.7 /
-# 647 / 655 / 660 (all terms); terms at most 1 / 2 / any: 623.5 / 621.8 /
-# 627.7 (no updates); none formed: 647.
+# LU entries the decomposition and the solves form instead of storing and
+# reading back (recomputed_l(), recomputed_u()): none that earlier
+# eliminations of its row update, at most RC_MAXTERMS Jacobian terms (the
+# _rc pair) or RC_MAXTERMS_WIDE (the _rcw pair, which the persistent Ros3
+# kernel runs).  Measured at 1M cells, default kernel: L only, updates
+# allowed 0 / 1 / 2 / 8: 627.7 / 647 / 655 / 660 ms (all terms); terms at
+# most 1 / 2 / any: 623.5 / 621.8 / 627.7 (no updates); none formed: 647.
+# With the U entries and the current build: terms at most 2 / 3 / 5:
+# 540.7 / 537.1 / 544.4 ms; one update allowed: 559.  The one-shot grid and
+# the family kernel are faster with 2 (775 against 798 ms; 2-5%).
 RC_MAXMOD = 0
 RC_MAXTERMS = 2
+RC_MAXTERMS_WIDE = 3
 
 
 def generate(dst=None, mech_path=None):
@@ -1072,6 +1078,8 @@ def generate(dst=None, mech_path=None):
     rc, nshared = gen_rconst(mech)
     rc_l = recomputed_l(mech, maxmod=RC_MAXMOD, maxterms=RC_MAXTERMS)
     rc_u = recomputed_u(mech, maxterms=RC_MAXTERMS)
+    rcw_l = recomputed_l(mech, maxmod=RC_MAXMOD, maxterms=RC_MAXTERMS_WIDE)
+    rcw_u = recomputed_u(mech, maxterms=RC_MAXTERMS_WIDE)
     dec, nfma = gen_decomp(mech, False)
     jdec, _ = gen_decomp(mech, True)
     parts = [
@@ -1142,6 +1150,12 @@ def generate(dst=None, mech_path=None):
         gen_jac_decomp_pf(mech, 16, rc_l | rc_u, "jac_decomp_rc_pf16", rc_u)[0],
         "",
         gen_solve_rc_pf_reg(mech, 16, rc_l, "solve_rc_pf16_reg", rc_u),
+        "",
+        "constexpr int RECOMPUTED_L_WIDE = %d;" % len(rcw_l),
+        "constexpr int RECOMPUTED_U_WIDE = %d;" % len(rcw_u),
+        gen_jac_decomp_pf(mech, 16, rcw_l | rcw_u, "jac_decomp_rcw_pf16", rcw_u)[0],
+        "",
+        gen_solve_rc_pf_reg(mech, 16, rcw_l, "solve_rcw_pf16_reg", rcw_u),
         "",
         gen_hot_tables(mech),
         "",

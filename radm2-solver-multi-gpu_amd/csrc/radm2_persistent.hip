@@ -104,7 +104,7 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_persistent(LaunchAr
 #endif
       cell_attempt_top(s, o);
       if (s.status == ST_RUNNING)
-        ros3_attempt<RADM2_LROWS, RADM2_METHODS != 0>(s, o, fc, kk, y, h2o, m, lu, f, k1, k2, k3);
+        ros3_attempt<RADM2_LROWS, RADM2_METHODS != 0, RADM2_METHODS == 0>(s, o, fc, kk, y, h2o, m, lu, f, k1, k2, k3);
     }
   }
   if (cell >= 0) {

@@ -198,17 +198,20 @@ __device__ __forceinline__ int stage_jac_decomp(KA k, YA y, double h2o, double m
 // and for U also by the rows that eliminate with them) from the same rate
 // constants, concentrations and inverse pivots, by the same operations:
 // the state and the rate constants they use (no photolysis rate, which the
-// forcing moves between stages) are fixed through an attempt.
-template <class KA, class YA, class GA>
+// forcing moves between stages) are fixed through an attempt.  WIDE: the
+// larger set (entries of up to three Jacobian terms instead of two).
+template <bool WIDE, class KA, class YA, class GA>
 __device__ __forceinline__ int stage_decomp(KA k, YA y, double h2o, double m, double ghinv,
                                             GA g) {
-  return gen::jac_decomp_rc_pf16(k, y, h2o, m, ghinv, g);
+  if constexpr (WIDE) return gen::jac_decomp_rcw_pf16(k, y, h2o, m, ghinv, g);
+  else return gen::jac_decomp_rc_pf16(k, y, h2o, m, ghinv, g);
 }
 
-template <class KA, class YA, class GA>
+template <bool WIDE, class KA, class YA, class GA>
 __device__ __forceinline__ void stage_solve_reg(KA k, YA y, double h2o, double m, GA g,
                                                 double (&x)[NVAR]) {
-  gen::solve_rc_pf16_reg(k, y, h2o, m, g, x);
+  if constexpr (WIDE) gen::solve_rcw_pf16_reg(k, y, h2o, m, g, x);
+  else gen::solve_rc_pf16_reg(k, y, h2o, m, g, x);
 }
 
 template <int RF, class KA, class VA, class FA>
@@ -433,7 +436,7 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
       launder(d);
       xpass<16>([&x, hg](int i, const double* v) { x[i] = fma(hg, v[0], x[i]); }, d);
     }
-    stage_solve_reg(k, y, h2o, m, g, x);
+    stage_solve_reg<false>(k, y, h2o, m, g, x);
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(ks, i, x[i]);
   }
@@ -491,15 +494,20 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
 // R: launder interval of the compiler-scheduled fused Jacobian/decomposition
 // (rows) and, as 16 reactions per unit, of the tendency on private arrays;
 // phases are laundered apart whenever the arrays are workspace views.
+// WIDE: the persistent Ros3 kernel's tuning of the fused stages (the wider
+// set of formed LU entries, and F2 + C31/H K1 kept for the third stage
+// instead of F2, which saves reloading K1 there); the one-shot grid and the
+// family kernel were faster without either (measured at 1M cells).
 // FAMILY: the kernel runs every method of KPP's Rosenbrock family, the
 // non-autonomous forcing fc and KPP's step halving after a singular matrix
 // (workspace views with seven stage regions); otherwise Ros3 alone,
 // autonomous, -8 at once, whatever o and fc ask.
-template <int R, bool FAMILY, class KA, class YA, class GA, class VA>
+template <int R, bool FAMILY, bool WIDE, class KA, class YA, class GA, class VA>
 __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, const Forcing& fc,
                                              KA k, YA y, double h2o, double m, GA g, VA f, VA k1,
                                              VA k2, VA k3) {
   static_assert(!FAMILY || is_ws_like<GA>::value, "the family needs workspace stage regions");
+  static_assert(!(FAMILY && WIDE), "the general stage loop solves with the narrower set");
   constexpr int RF = R > 0 ? 16 * R : 0;
   double H = s.H;
   const bool forced = FAMILY && fc.j1 != nullptr;
@@ -538,7 +546,8 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
   if constexpr (is_ws_like<GA>::value) {
     // Fused stages: each right-hand side is formed in registers and solved
     // there; K1 and K2 are stored once, K3 never (the combination uses it
-    // in place), and F2 once for the third stage.  The stage-2
+    // in place), and F2 (or, WIDE, F2 + C31/H K1) once for the third
+    // stage.  The stage-2
     // state y + K1 is formed where the tendency reads it (never stored),
     // the third right-hand side takes K2 from the registers the second
     // solve leaves it in, and on the Ros3 kernels Ynew stays in registers
@@ -551,7 +560,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
       // code: none, mgpu.cu:3147-3163), then -8
       if (forced) photo_at(k, fc, s.T);
       for (int nsc = 0;; nsc++) {
-        if (stage_decomp(k, y, h2o, m, 1.0 / (H * gam), g) == 0) break;
+        if (stage_decomp<WIDE>(k, y, h2o, m, 1.0 / (H * gam), g) == 0) break;
         if (nsc >= o.sing_retries) {
           s.H = H;
           s.status = ST_SINGULAR;
@@ -563,7 +572,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
       hc31 = ROS_C31 / H;
       hc32 = ROS_C32 / H;
     } else {
-      if (stage_decomp(k, y, h2o, m, ghinv, g) != 0) {
+      if (stage_decomp<WIDE>(k, y, h2o, m, ghinv, g) != 0) {
         s.status = ST_SINGULAR;
         return;
       }
@@ -624,7 +633,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
     } else {
       gen::fun_pf16_reg(k, y, h2o, m, x);
     }
-    stage_solve_reg(k, y, h2o, m, g, x);
+    stage_solve_reg<WIDE>(k, y, h2o, m, g, x);
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(k1, i, x[i]);
     // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
@@ -633,30 +642,44 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
       if (fused_forced) photo_at(k, fc, s.T + r3.alpha[1] * H);
     }
     gen::fun_pf16_reg(k, SumView<YA, VA>{y, k1}, h2o, m, x);
+    if constexpr (WIDE) {
+      // with K1 loaded: the second right-hand side F2 + C21/H K1, and the
+      // inner term of the third's, F2 + C31/H K1, kept in f
+      xpass<16>([&x, f, hc21, hc31](int i, const double* a) {
+        const double f2 = x[i];
+        WR(f, i, fma(a[0], hc31, f2));
+        x[i] = fma(a[0], hc21, f2);
+      }, k1);
+    } else {
 #pragma unroll
-    for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
-    xpass<16>([&x, hc21](int i, const double* a) { x[i] = fma(a[0], hc21, x[i]); }, k1);
+      for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
+      xpass<16>([&x, hc21](int i, const double* a) { x[i] = fma(a[0], hc21, x[i]); }, k1);
+    }
     if constexpr (FAMILY) {
       if (fused_forced) {
         const double hg = H * r3.gamma[1];
         xpass<16>([&x, hg](int i, const double* a) { x[i] = fma(hg, a[0], x[i]); }, dfdt);
       }
     }
-    stage_solve_reg(k, y, h2o, m, g, x);
+    stage_solve_reg<WIDE>(k, y, h2o, m, g, x);
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(k2, i, x[i]);
-    // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2), K2 still in
-    // registers   (forced: + H gamma_3 dF/dT)
-    xpass<16>([&x, hc31, hc32](int i, const double* a) {
-      x[i] = fma(x[i], hc32, fma(a[0], hc31, a[1]));
-    }, k1, f);
+    // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2), with K2
+    // still in registers   (forced: + H gamma_3 dF/dT)
+    if constexpr (WIDE) {
+      xpass<16>([&x, hc32](int i, const double* a) { x[i] = fma(x[i], hc32, a[0]); }, f);
+    } else {
+      xpass<16>([&x, hc31, hc32](int i, const double* a) {
+        x[i] = fma(x[i], hc32, fma(a[0], hc31, a[1]));
+      }, k1, f);
+    }
     if constexpr (FAMILY) {
       if (fused_forced) {
         const double hg = H * r3.gamma[2];
         xpass<16>([&x, hg](int i, const double* a) { x[i] = fma(hg, a[0], x[i]); }, dfdt);
       }
     }
-    stage_solve_reg(k, y, h2o, m, g, x);
+    stage_solve_reg<WIDE>(k, y, h2o, m, g, x);
     // combination and WRMS error norm with K3 in registers; Ynew replaces
     // K3 in x on the Ros3 kernels (written to y on acceptance), in f on the
     // family's (whose other methods leave it there)

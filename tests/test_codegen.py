@@ -76,16 +76,20 @@ def _body(header, sig):
     return header[start:header.index("\n}\n", start)]
 
 
-def _formed():
-    return (G.recomputed_l(MECH, maxmod=G.RC_MAXMOD, maxterms=G.RC_MAXTERMS),
-            G.recomputed_u(MECH, maxterms=G.RC_MAXTERMS))
+def _formed(wide=False):
+    t = G.RC_MAXTERMS_WIDE if wide else G.RC_MAXTERMS
+    return (G.recomputed_l(MECH, maxmod=G.RC_MAXMOD, maxterms=t),
+            G.recomputed_u(MECH, maxterms=t))
 
 
-@pytest.mark.parametrize("fn", ["jac_decomp_pf16", "jac_decomp_rc_pf16"])
+PAIRS = [("rc", False, ""), ("rcw", True, "_WIDE")]
+
+
+@pytest.mark.parametrize("fn", ["jac_decomp_pf16", "jac_decomp_rc_pf16", "jac_decomp_rcw_pf16"])
 def test_lu_entries_are_read_after_they_are_stored(header, fn):
     body = _body(header, "int %s(" % fn)
-    rc_l, rc_u = _formed()
-    skipped = {"jac_decomp_pf16": frozenset(), "jac_decomp_rc_pf16": rc_l | rc_u}[fn]
+    skipped = frozenset() if fn == "jac_decomp_pf16" else frozenset().union(
+        *_formed(wide=(fn == "jac_decomp_rcw_pf16")))
     stored = {}
     assert len(re.findall(r"RDP<\d+>\(g\)", body)) > 300   # the check is not vacuous
     for n, ln in enumerate(body.split("\n")):
@@ -100,7 +104,8 @@ def test_lu_entries_are_read_after_they_are_stored(header, fn):
     assert sorted(stored) == sorted(set(range(MECH["lu"]["nnz"])) - skipped)
 
 
-def test_recomputed_l_entries_are_formed_as_the_decomposition_forms_them(header):
+@pytest.mark.parametrize("pair,wide,suffix", PAIRS)
+def test_recomputed_l_entries_are_formed_as_the_decomposition_forms_them(header, pair, wide, suffix):
     # the solves form these L entries instead of reading them back: each
     # must be an L position that no earlier elimination of its row updates,
     # never stored by the decomposition the kernels run, never read by their
@@ -108,11 +113,11 @@ def test_recomputed_l_entries_are_formed_as_the_decomposition_forms_them(header)
     # the same inverse pivot
     lu = MECH["lu"]
     crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
-    rc, rc_u = _formed()
-    assert len(rc) == int(re.search(r"RECOMPUTED_L = (\d+);", header).group(1)) > 50
+    rc, rc_u = _formed(wide)
+    assert len(rc) == int(re.search(r"RECOMPUTED_L%s = (\d+);" % suffix, header).group(1)) > 50
     row_of = {kk: k for k in range(59) for kk in range(crow[k], crow[k + 1])}
     dec = _body(header, "int jac_decomp_pf16(")
-    solve = _body(header, "void solve_rc_pf16_reg(")
+    solve = _body(header, "void solve_%s_pf16_reg(" % pair)
     read = {int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", solve)}
     assert not (read & rc)
     # every other LU entry is read (the formed U entries are formed too)
@@ -178,7 +183,8 @@ def test_photolysis_map_matches_the_rate_laws():
     assert sorted(pj) == list(range(21))
 
 
-def test_recomputed_u_entries_are_formed_as_the_decomposition_forms_them(header):
+@pytest.mark.parametrize("pair,wide,suffix", PAIRS)
+def test_recomputed_u_entries_are_formed_as_the_decomposition_forms_them(header, pair, wide, suffix):
     # the decomposition and the solves also form U entries: each an off-diagonal U
     # position that no elimination of its row updates, so that it is the
     # Jacobian expression itself; the decomposition neither stores nor reads
@@ -186,12 +192,12 @@ def test_recomputed_u_entries_are_formed_as_the_decomposition_forms_them(header)
     # backward substitution
     lu = MECH["lu"]
     crow, diag, icol = lu["crow"], lu["diag"], lu["icol"]
-    rc_l, rc_u = _formed()
-    assert len(rc_u) == int(re.search(r"RECOMPUTED_U = (\d+);", header).group(1)) > 20
+    rc_l, rc_u = _formed(wide)
+    assert len(rc_u) == int(re.search(r"RECOMPUTED_U%s = (\d+);" % suffix, header).group(1)) > 20
     row_of = {kk: k for k in range(59) for kk in range(crow[k], crow[k + 1])}
     dec0 = _body(header, "int jac_decomp_pf16(")
-    dec = _body(header, "int jac_decomp_rc_pf16(")
-    solve = _body(header, "void solve_rc_pf16_reg(")
+    dec = _body(header, "int jac_decomp_%s_pf16(" % pair)
+    solve = _body(header, "void solve_%s_pf16_reg(" % pair)
     read = {int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", solve)}
     assert read == set(range(lu["nnz"])) - rc_l - rc_u
     assert not ({int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", dec)} & rc_u)
