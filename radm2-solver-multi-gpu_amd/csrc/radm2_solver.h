@@ -243,6 +243,28 @@ __device__ __forceinline__ double RD(const SumView<YA, KA>& v, int i) {
   return RD(v.y, i) + RD(v.k, i);
 }
 
+// The stage-2 state y + a K1 of a method whose A(2,1) = a is not 1 (Ros2),
+// read the same way: entry i is fma(a, RD(k1, i), RD(y, i)), the operation
+// the general stage loop forms that state with.
+template <class YA, class KA>
+struct AxpyView {
+  YA y;
+  KA k;
+  double a;
+};
+struct AxpyRead {
+  double y, k, a;
+};
+template <int I, class YA, class KA>
+__device__ __forceinline__ AxpyRead RDP(const AxpyView<YA, KA>& v) {
+  return AxpyRead{V(RDP<I>(v.y)), RD(v.k, I), v.a};
+}
+__device__ __forceinline__ double V(const AxpyRead& r) { return fma(r.a, r.k, r.y); }
+template <class YA, class KA>
+__device__ __forceinline__ double RD(const AxpyView<YA, KA>& v, int i) {
+  return fma(v.a, RD(v.k, i), RD(v.y, i));
+}
+
 // Element-wise pass out[i] = fn(i, {in0[i], in1[i], ..}) over the 59
 // entries, CH at a time: all inputs of a chunk are loaded before any of its
 // outputs is stored, and the next chunk's inputs before this chunk's stores
@@ -352,8 +374,9 @@ __device__ __forceinline__ double forcing_delta(double T, double dt) {
 }
 
 // Stages of one attempt of KPP's generic Rosenbrock scheme on workspace
-// views (the methods other than Ros3 and, when forced, Ros3 in a wave whose
-// other cells are not all forced Ros3): stage
+// views (the methods other than Ros3 and Ros2; forced Ros3 in a wave whose
+// other cells are not all forced Ros3; forced Ros2, and autonomous Ros2 in a
+// wave whose other cells are not all autonomous Ros2): stage
 // s's vector lives in the workspace region k1.region + s*NVAR, dF/dT in
 // region 6, the latest new F in f.  The reference's commented stage loop
 // (mgpu.cu:3595-3625) in KPP's general form: F = f(T, y) for stage 0,
@@ -518,6 +541,15 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
   bool fused_forced = false;
   if constexpr (FAMILY) fused_forced = __ballot(forced && o.method == 2) == __ballot(true);
   const bool ros3 = !FAMILY || (o.method == 2 && (!forced || fused_forced));
+  // autonomous Ros2 runs the first two fused stages with its own
+  // coefficients, then its own combination (its two stages have Ros3's
+  // shape: a new F in each, the second at y + A21 K1), when every active
+  // lane of the wave is an autonomous Ros2 cell (wave-uniform, as above; a
+  // mixed wave sends its Ros2 cells through the general stage loop, with
+  // identical results)
+  bool two = false;
+  if constexpr (FAMILY) two = __ballot(o.method == 1 && !forced) == __ballot(true);
+  constexpr RosMethod r2 = kRosMethods[1];
   // the family reads its scalars from the table for every lane (Ros3's row
   // holds Ros3's values): a per-lane choice between a constant and a load
   // would be a branch whose merged value the compiler may spill under the
@@ -526,7 +558,8 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
   const double elo = FAMILY ? rm.elo : ROS_ELO;
   const double gam = FAMILY ? rm.gamma1 : ROS_GAMMA1;
   const double ghinv = 1.0 / (H * gam);
-  double hc21 = ROS_C21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
+  const double c21 = two ? r2.C[0] : ROS_C21;
+  double hc21 = c21 / H, hc31 = ROS_C31 / H, hc32 = ROS_C32 / H;
   const double rtol = o.rtol, atol = o.atol;
   // On the Ros3 kernels the step controller's options are read at the top
   // with the others (read at their use they would wait a memory round trip
@@ -551,7 +584,9 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
     // state y + K1 is formed where the tendency reads it (never stored),
     // the third right-hand side takes K2 from the registers the second
     // solve leaves it in, and on the Ros3 kernels Ynew stays in registers
-    // until the step is accepted.  The decomposition does not depend on
+    // until the step is accepted.  Autonomous Ros2 runs the first two of
+    // these stages with its own A21 and C21 and combines with K2 in
+    // registers (K2 never stored).  The decomposition does not depend on
     // F0, so it comes first.
     // G = -J(y) + ghinv*I formed row by row inside its own LU decomposition
     if constexpr (FAMILY) {
@@ -568,7 +603,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
         }
         H = H * 0.5;
       }
-      hc21 = ROS_C21 / H;
+      hc21 = c21 / H;
       hc31 = ROS_C31 / H;
       hc32 = ROS_C32 / H;
     } else {
@@ -577,7 +612,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
         return;
       }
     }
-    if (!ros3) {
+    if (!ros3 && !two) {
       // one method at a time, the same for every lane that runs it, so that
       // its coefficients are read through the scalar cache (a wave whose
       // cells share a method, the usual case, takes one round)
@@ -637,11 +672,16 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
 #pragma unroll
     for (int i = 0; i < NVAR; i++) WR(k1, i, x[i]);
     // stage 2: F2 = f(y + K1); K2 = G^-1 (F2 + C21/H K1)
-    //   (forced: F2 at T + alpha_2 H, + H gamma_2 dF/dT)
+    //   (forced: F2 at T + alpha_2 H, + H gamma_2 dF/dT; Ros2: f(y + A21 K1))
     if constexpr (FAMILY) {
       if (fused_forced) photo_at(k, fc, s.T + r3.alpha[1] * H);
     }
-    gen::fun_pf16_reg(k, SumView<YA, VA>{y, k1}, h2o, m, x);
+    if constexpr (FAMILY) {
+      // y + K1 as fma(1, K1, y), the same value, or Ros2's y + A21 K1
+      gen::fun_pf16_reg(k, AxpyView<YA, VA>{y, k1, two ? r2.A[0] : 1.0}, h2o, m, x);
+    } else {
+      gen::fun_pf16_reg(k, SumView<YA, VA>{y, k1}, h2o, m, x);
+    }
     if constexpr (WIDE) {
       // with K1 loaded: the second right-hand side F2 + C21/H K1, and the
       // inner term of the third's, F2 + C31/H K1, kept in f
@@ -651,8 +691,10 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
         x[i] = fma(a[0], hc21, f2);
       }, k1);
     } else {
+      if (!two) {
 #pragma unroll
-      for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
+        for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
+      }
       xpass<16>([&x, hc21](int i, const double* a) { x[i] = fma(a[0], hc21, x[i]); }, k1);
     }
     if constexpr (FAMILY) {
@@ -662,38 +704,53 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
       }
     }
     stage_solve_reg<WIDE>(k, y, h2o, m, g, x);
-#pragma unroll
-    for (int i = 0; i < NVAR; i++) WR(k2, i, x[i]);
-    // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2), with K2
-    // still in registers   (forced: + H gamma_3 dF/dT)
-    if constexpr (WIDE) {
-      xpass<16>([&x, hc32](int i, const double* a) { x[i] = fma(x[i], hc32, a[0]); }, f);
+    if (two) {
+      // Ros2's combination and error norm with K2 in registers, in the
+      // general stage loop's order; Ynew in f
+      xpass<8>([&x, &err, f, rtol, atol](int i, const double* a) {
+        const double a1 = a[0], yi = a[1];
+        const double sm = fma(r2.M[1], x[i], r2.M[0] * a1);
+        const double e = fma(r2.E[1], x[i], r2.E[0] * a1);
+        const double yn = yi + sm;
+        const double ymax = dmax_ref(fabs(yi), fabs(yn));
+        const double q = e / fma(rtol, ymax, atol);
+        err = fma(q, q, err);
+        WR(f, i, yn);
+      }, k1, y);
     } else {
-      xpass<16>([&x, hc31, hc32](int i, const double* a) {
-        x[i] = fma(x[i], hc32, fma(a[0], hc31, a[1]));
-      }, k1, f);
-    }
-    if constexpr (FAMILY) {
-      if (fused_forced) {
-        const double hg = H * r3.gamma[2];
-        xpass<16>([&x, hg](int i, const double* a) { x[i] = fma(hg, a[0], x[i]); }, dfdt);
+#pragma unroll
+      for (int i = 0; i < NVAR; i++) WR(k2, i, x[i]);
+      // stage 3 reuses F2: K3 = G^-1 (F2 + C31/H K1 + C32/H K2), with K2
+      // still in registers   (forced: + H gamma_3 dF/dT)
+      if constexpr (WIDE) {
+        xpass<16>([&x, hc32](int i, const double* a) { x[i] = fma(x[i], hc32, a[0]); }, f);
+      } else {
+        xpass<16>([&x, hc31, hc32](int i, const double* a) {
+          x[i] = fma(x[i], hc32, fma(a[0], hc31, a[1]));
+        }, k1, f);
       }
+      if constexpr (FAMILY) {
+        if (fused_forced) {
+          const double hg = H * r3.gamma[2];
+          xpass<16>([&x, hg](int i, const double* a) { x[i] = fma(hg, a[0], x[i]); }, dfdt);
+        }
+      }
+      stage_solve_reg<WIDE>(k, y, h2o, m, g, x);
+      // combination and WRMS error norm with K3 in registers; Ynew replaces
+      // K3 in x on the Ros3 kernels (written to y on acceptance), in f on the
+      // family's (whose other methods leave it there)
+      xpass<8>([&x, &err, f, rtol, atol](int i, const double* a) {
+        const double a1 = a[0], a2 = a[1], a3 = x[i], yi = a[2];
+        const double sm = fma(ROS_M3, a3, fma(ROS_M2, a2, ROS_M1 * a1));
+        const double yn = yi + sm;
+        const double e = fma(ROS_E3, a3, fma(ROS_E2, a2, ROS_E1 * a1));
+        const double ymax = dmax_ref(fabs(yi), fabs(yn));
+        const double q = e / fma(rtol, ymax, atol);
+        err = fma(q, q, err);
+        if constexpr (FAMILY) WR(f, i, yn);
+        else x[i] = yn;
+      }, k1, k2, y);
     }
-    stage_solve_reg<WIDE>(k, y, h2o, m, g, x);
-    // combination and WRMS error norm with K3 in registers; Ynew replaces
-    // K3 in x on the Ros3 kernels (written to y on acceptance), in f on the
-    // family's (whose other methods leave it there)
-    xpass<8>([&x, &err, f, rtol, atol](int i, const double* a) {
-      const double a1 = a[0], a2 = a[1], a3 = x[i], yi = a[2];
-      const double sm = fma(ROS_M3, a3, fma(ROS_M2, a2, ROS_M1 * a1));
-      const double yn = yi + sm;
-      const double e = fma(ROS_E3, a3, fma(ROS_E2, a2, ROS_E1 * a1));
-      const double ymax = dmax_ref(fabs(yi), fabs(yn));
-      const double q = e / fma(rtol, ymax, atol);
-      err = fma(q, q, err);
-      if constexpr (FAMILY) WR(f, i, yn);
-      else x[i] = yn;
-    }, k1, k2, y);
     }
   } else {
     // F0 = f(y); K1 = G^-1 F0

@@ -314,6 +314,56 @@ def test_forced_ros3_fused_stages_match_the_general_loop(solver):
     solver.set_schedule(radm2_amd.SCHED_PERSISTENT)
 
 
+def test_ros2_fused_stages_match_the_general_loop(solver):
+    # autonomous Ros2 runs the first two fused stages and its own combination
+    # when every active lane of a wave is an autonomous Ros2 cell, else the
+    # general stage loop.  One Ros2 domain (whole waves: the fused stages)
+    # against the same cells as one-cell domains alternating with Ros3 cells
+    # (mixed waves: the general loop), bit for bit, on both schedules; the
+    # oracle's step counts and end states within 1e-6 for the first
+    import radm2_amd
+    from radm2_amd import synth
+    n = 96
+    ca = synth.make_cells(n, 91, 100.0, 0.05)
+    cb = synth.make_cells(n, 92, 100.0, 0.05)
+    keys = ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")
+
+    def batch(c):
+        d = synth.to_device(c)
+        return radm2_amd.CellBatch(d["conc"], d["temp"], d["cair"], d["h2o"], d["jrate"],
+                                   rc_n2o5=d["rc_n2o5"])
+
+    def cell(c, i):
+        return {k: np.ascontiguousarray(c[k][..., i:i + 1]) for k in keys}
+
+    o2, o3 = radm2_amd.Opts(method=1), radm2_amd.Opts()
+    co, oo, code = O.integrate(ca, 600.0, 1e-3, 1.0, opts=O.OptsIn(method=1), nthreads=0)
+    assert code == 0 and (oo["status"] == 1).all()
+    for sched in (radm2_amd.SCHED_PERSISTENT, radm2_amd.SCHED_GRID):
+        solver.set_schedule(sched)
+        whole = batch(ca)
+        solver.integrate(whole, 600.0, 1e-3, 1.0, opts=o2)
+        singles, others, doms = [], [], []   # every batch held for the call
+        for i in range(n):
+            a, b = batch(cell(ca, i)), batch(cell(cb, i))
+            singles.append(a)
+            others.append(b)
+            doms += [a.domain(600.0, 1e-3, 1.0, opts=o2), b.domain(600.0, 1e-3, 1.0, opts=o3)]
+        solver.integrate_batched(doms)
+        solver.sync()
+        assert (whole.status.cpu().numpy() == 1).all()
+        assert all((x.status.cpu().numpy() == 1).all() for x in others)
+        for key in ("conc", "nstep", "nacc", "nrej", "status", "hexit"):
+            got = np.concatenate([getattr(a, key).cpu().numpy() for a in singles], axis=-1)
+            np.testing.assert_array_equal(got, getattr(whole, key).cpu().numpy(),
+                                          err_msg="%s sched %d" % (key, sched))
+        og = dict(nstep=whole.nstep.cpu().numpy(), nacc=whole.nacc.cpu().numpy(),
+                  nrej=whole.nrej.cpu().numpy())
+        same, ok, rel = _compare(whole.conc.cpu().numpy(), og, co, oo, "Ros2 sched %d" % sched)
+        assert ok.all() and same.all()
+    solver.set_schedule(radm2_amd.SCHED_PERSISTENT)
+
+
 def test_family_plan_replays_match_eager(solver):
     # a captured plan of a forced Rodas3 domain replays bit-identically to the
     # eager call, with eager work between the replays
