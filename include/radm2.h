@@ -119,8 +119,9 @@ int radm2_set_schedule(radm2_ctx *ctx, int schedule, int block, int refill_min);
  * (an explicit wave-interleaved HBM workspace whose loads are issued in
  * batches ahead of their use), RADM2_PLACE_LDS160 / RADM2_PLACE_LDS312 (the
  * workspace with its 160 / 312 most accessed doubles per lane held in LDS),
- * RADM2_PLACE_LDS160_REG128 (160 in LDS and the next 128 in registers; the
- * default) -- the LDS placements run 64-lane blocks only, other block sizes
+ * RADM2_PLACE_LDS160_REG112 (160 in LDS and the next 112 in registers; the
+ * default; RADM2_PLACE_LDS160_REG128 is its former name, from when the
+ * register tier held 128) -- the LDS placements run 64-lane blocks only, other block sizes
  * give RADM2_E_ARG at launch;
  * -1 = tuned default -- and the register budget as the minimum resident
  * waves per SIMD (1, 2, 4 or 8; 0 = tuned default).  Returns
@@ -130,7 +131,8 @@ int radm2_set_schedule(radm2_ctx *ctx, int schedule, int block, int refill_min);
 #define RADM2_PLACE_WORKSPACE 1
 #define RADM2_PLACE_LDS160 2
 #define RADM2_PLACE_LDS312 3
-#define RADM2_PLACE_LDS160_REG128 4
+#define RADM2_PLACE_LDS160_REG112 4
+#define RADM2_PLACE_LDS160_REG128 RADM2_PLACE_LDS160_REG112
 /* A cell spread over 2 (SPLIT2) or 4 (SPLIT4) lanes that all run its
  * computation: the workspace plus, per cell, 160 / 320 doubles in LDS and
  * 256 / 512 in registers split over the cell's lanes (persistent schedule,

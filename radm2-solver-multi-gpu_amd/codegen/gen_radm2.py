@@ -508,7 +508,7 @@ def gen_solve_pf(mech, G):
     return "\n".join(out)
 
 
-def recomputed_l(mech, S=160, R=128, P=0, L=1, maxmod=0, maxterms=1 << 30):
+def recomputed_l(mech, S=160, R=112, P=0, L=1, maxmod=0, maxterms=1 << 30):
     """L positions (row k, column j < k) that the solves re-form instead of
     reading back, with the decomposition's own operations: L(k, j) =
     (Jacobian expression - sum over the row's earlier eliminations i that
@@ -549,7 +549,7 @@ def recomputed_l(mech, S=160, R=128, P=0, L=1, maxmod=0, maxterms=1 << 30):
     return frozenset(out)
 
 
-def recomputed_u(mech, S=160, R=128, P=0, L=1, maxterms=1 << 30):
+def recomputed_u(mech, S=160, R=112, P=0, L=1, maxterms=1 << 30):
     """Off-diagonal U positions (row k, column c > k) left in the workspace
     under hot set (S, R, P, L) that no elimination of their row updates:
     U(k, c) is then the Jacobian expression itself.  Eligible when that
@@ -977,7 +977,7 @@ def hot_ranking(mech):
 # of a cell split over its L lanes (register r of lane q of the cell holds
 # split entry r*L + q, so a cell holds P*L entries there); L = 1: one cell
 # per lane
-HOT_BUDGETS = ((160, 0, 0, 1), (312, 0, 0, 1), (160, 128, 0, 1), (160, 0, 128, 2),
+HOT_BUDGETS = ((160, 0, 0, 1), (312, 0, 0, 1), (160, 112, 0, 1), (160, 0, 128, 2),
                (320, 0, 128, 4))
 # code of a split-tier entry: SPLIT_BASE - (its index in the split tier);
 # of a workspace entry: HBM_BASE - (its rank among the workspace entries,
@@ -1069,6 +1069,18 @@ def gen_wrf_tables(mech):
 RC_MAXMOD = 0
 RC_MAXTERMS = 2
 RC_MAXTERMS_WIDE = 3
+# Load batch of the _rcw pair and of the tendency every kernel runs in
+# registers (fun_pf20_reg), with the default placement's register tier at
+# 112 doubles: measured at 1M cells, default kernel, batches 16 / 18 / 20 /
+# 22 / 24 with 112 registers 532 (16 at 128) / 525 / 519 / 531 / 526 ms; 20
+# with 104 / 108 / 116 / 120 registers 526 / 526 / 534 / 548; 20 with one
+# of the decomposition, the solves or the tendency at 16 or 24: 523-535;
+# 24 with 96 registers 522, 32 with 64 registers 525 (all bit-identical).
+# The family kernel: the tendency at 20 against 16, Ros4, Rodas3 and Rodas4
+# 2.3% faster, Ros2 1.2% and forced Ros3 0.4% slower; 20 in the general
+# stage loop only was slower on every method.
+RC_BATCH_WIDE = 20
+FUN_REG_BATCH = 20
 
 
 def generate(dst=None, mech_path=None):
@@ -1142,7 +1154,7 @@ def generate(dst=None, mech_path=None):
         "",
         gen_solve_pf_reg(mech, 16),
         "",
-        gen_fun_pf_reg(mech, 16),
+        gen_fun_pf_reg(mech, FUN_REG_BATCH),
         "",
         "// L and U entries formed at their use instead of stored and read back",
         "constexpr int RECOMPUTED_L = %d;" % len(rc_l),
@@ -1153,9 +1165,9 @@ def generate(dst=None, mech_path=None):
         "",
         "constexpr int RECOMPUTED_L_WIDE = %d;" % len(rcw_l),
         "constexpr int RECOMPUTED_U_WIDE = %d;" % len(rcw_u),
-        gen_jac_decomp_pf(mech, 16, rcw_l | rcw_u, "jac_decomp_rcw_pf16", rcw_u)[0],
+        gen_jac_decomp_pf(mech, RC_BATCH_WIDE, rcw_l | rcw_u, "jac_decomp_rcw_pf%d" % RC_BATCH_WIDE, rcw_u)[0],
         "",
-        gen_solve_rc_pf_reg(mech, 16, rcw_l, "solve_rcw_pf16_reg", rcw_u),
+        gen_solve_rc_pf_reg(mech, RC_BATCH_WIDE, rcw_l, "solve_rcw_pf%d_reg" % RC_BATCH_WIDE, rcw_u),
         "",
         gen_hot_tables(mech),
         "",

@@ -81,12 +81,12 @@ constexpr int kDefaultPersistentBlock = 64;
 // Defaults measured on MI355X at 1M cells (the autotuner may override):
 // persistent schedule, explicit workspace placement with its load-scheduled
 // kernels, the 160 most accessed doubles per lane in LDS (two 64-lane
-// blocks per CU) and the next 128 in registers (1 wave per SIMD register
-// budget), refill once two lanes of a wave are idle (1: 537.5 ms, 2: 531.7,
-// 3: 534.1 at 1M cells).
+// blocks per CU) and the next 112 in registers (1 wave per SIMD register
+// budget; 128 left no room for load batches of 20), refill once two lanes
+// of a wave are idle (1: 537.5 ms, 2: 531.7, 3: 534.1 at 1M cells).
 constexpr int kDefaultRefillMin = 2;
 constexpr int kDefaultWaves = 1;
-constexpr int kDefaultPlacement = radm2::PLACE_HOT160R128;
+constexpr int kDefaultPlacement = radm2::PLACE_HOT160R112;
 
 // The configured kernel variant; family: one that runs all of KPP's
 // Rosenbrock methods (built for the default placement only).

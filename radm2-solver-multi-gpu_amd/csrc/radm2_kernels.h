@@ -77,15 +77,16 @@ constexpr int PLACE_HOT312 = RADM2_PLACE_LDS312;
 // per lane, split register entries per lane, lanes per cell) of each
 // placement; measured at 1M cells with one cell per lane: more LDS per
 // lane (fewer resident waves) or more registers (spills) or fewer LDS
-// slots (more waves, more HBM traffic) are all slower than (160, 128).
+// slots (more waves, more HBM traffic) are all slower than (160, 112), which
+// leaves the Ros3 kernels the registers for load batches of 20.
 // The split placements spread a cell over 2 or 4 lanes so that four waves
 // per CU (one per SIMD) hold as many cells (2 lanes) or half as many (4
 // lanes) as the one-lane placements' two.
-constexpr int PLACE_HOT160R128 = RADM2_PLACE_LDS160_REG128;
+constexpr int PLACE_HOT160R112 = RADM2_PLACE_LDS160_REG112;
 constexpr int PLACE_SPLIT2 = RADM2_PLACE_SPLIT2;
 constexpr int PLACE_SPLIT4 = RADM2_PLACE_SPLIT4;
 constexpr int kHotConfig[7][4] = {{0, 0, 0, 1},   {0, 0, 0, 1},     {160, 0, 0, 1},
-                                  {312, 0, 0, 1}, {160, 128, 0, 1}, {160, 0, 128, 2},
+                                  {312, 0, 0, 1}, {160, 112, 0, 1}, {160, 0, 128, 2},
                                   {320, 0, 128, 4}};
 constexpr int PLACE_HOT_LAST = PLACE_SPLIT4;
 constexpr int hot_slots(int placement) { return kHotConfig[placement][0]; }

@@ -199,18 +199,19 @@ __device__ __forceinline__ int stage_jac_decomp(KA k, YA y, double h2o, double m
 // constants, concentrations and inverse pivots, by the same operations:
 // the state and the rate constants they use (no photolysis rate, which the
 // forcing moves between stages) are fixed through an attempt.  WIDE: the
-// larger set (entries of up to three Jacobian terms instead of two).
+// larger set (entries of up to three Jacobian terms instead of two), its
+// loads issued 20 at a time instead of 16.
 template <bool WIDE, class KA, class YA, class GA>
 __device__ __forceinline__ int stage_decomp(KA k, YA y, double h2o, double m, double ghinv,
                                             GA g) {
-  if constexpr (WIDE) return gen::jac_decomp_rcw_pf16(k, y, h2o, m, ghinv, g);
+  if constexpr (WIDE) return gen::jac_decomp_rcw_pf20(k, y, h2o, m, ghinv, g);
   else return gen::jac_decomp_rc_pf16(k, y, h2o, m, ghinv, g);
 }
 
 template <bool WIDE, class KA, class YA, class GA>
 __device__ __forceinline__ void stage_solve_reg(KA k, YA y, double h2o, double m, GA g,
                                                 double (&x)[NVAR]) {
-  if constexpr (WIDE) gen::solve_rcw_pf16_reg(k, y, h2o, m, g, x);
+  if constexpr (WIDE) gen::solve_rcw_pf20_reg(k, y, h2o, m, g, x);
   else gen::solve_rc_pf16_reg(k, y, h2o, m, g, x);
 }
 
@@ -422,7 +423,7 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
       for (int i = 0; i < NVAR; i++) WR(ks, i, x[i]);
       launder(ks);
       if (forced && st > 0) photo_at(k, fc, T + rm.alpha[st] * H);
-      gen::fun_pf16_reg(k, ks, h2o, m, x);
+      gen::fun_pf20_reg(k, ks, h2o, m, x);
 #pragma unroll
       for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
       if (st == 0 && forced) {
@@ -430,7 +431,7 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
         const double delta = forcing_delta(T, fc.dt);
         const double rd = 1.0 / delta;
         photo_at(k, fc, T + delta);
-        gen::fun_pf16_reg(k, ks, h2o, m, x);
+        gen::fun_pf20_reg(k, ks, h2o, m, x);
         VA f0 = f;
         launder(f0);
 #pragma unroll
@@ -645,7 +646,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
       const double delta = forcing_delta(s.T, fc.dt);
 #pragma nounroll
       for (int pass = 0;; pass++) {
-        gen::fun_pf16_reg(k, y, h2o, m, x);
+        gen::fun_pf20_reg(k, y, h2o, m, x);
         if (!fused_forced || pass > 0) break;
 #pragma unroll
         for (int i = 0; i < NVAR; i++) WR(f, i, x[i]);
@@ -666,7 +667,7 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
         }
       }
     } else {
-      gen::fun_pf16_reg(k, y, h2o, m, x);
+      gen::fun_pf20_reg(k, y, h2o, m, x);
     }
     stage_solve_reg<WIDE>(k, y, h2o, m, g, x);
 #pragma unroll
@@ -678,9 +679,9 @@ __device__ __forceinline__ void ros3_attempt(CellState& s, const Ros3Opts& o, co
     }
     if constexpr (FAMILY) {
       // y + K1 as fma(1, K1, y), the same value, or Ros2's y + A21 K1
-      gen::fun_pf16_reg(k, AxpyView<YA, VA>{y, k1, two ? r2.A[0] : 1.0}, h2o, m, x);
+      gen::fun_pf20_reg(k, AxpyView<YA, VA>{y, k1, two ? r2.A[0] : 1.0}, h2o, m, x);
     } else {
-      gen::fun_pf16_reg(k, SumView<YA, VA>{y, k1}, h2o, m, x);
+      gen::fun_pf20_reg(k, SumView<YA, VA>{y, k1}, h2o, m, x);
     }
     if constexpr (WIDE) {
       // with K1 loaded: the second right-hand side F2 + C21/H K1, and the

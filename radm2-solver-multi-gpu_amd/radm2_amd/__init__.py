@@ -36,7 +36,8 @@ PLACE_PRIVATE = 0
 PLACE_WORKSPACE = 1
 PLACE_LDS160 = 2
 PLACE_LDS312 = 3
-PLACE_LDS160_REG128 = 4
+PLACE_LDS160_REG112 = 4
+PLACE_LDS160_REG128 = PLACE_LDS160_REG112   # former name
 PLACE_SPLIT2 = 5
 PLACE_SPLIT4 = 6
 

@@ -30,17 +30,17 @@ import json
 import os
 import time
 
-from . import (PLACE_LDS160, PLACE_LDS160_REG128, PLACE_LDS312, PLACE_PRIVATE, PLACE_SPLIT2,
+from . import (PLACE_LDS160, PLACE_LDS160_REG112, PLACE_LDS312, PLACE_PRIVATE, PLACE_SPLIT2,
                PLACE_SPLIT4, PLACE_WORKSPACE, SCHED_GRID, SCHED_PERSISTENT, Radm2Error)
 
 KERNELS = [(PLACE_PRIVATE, 1), (PLACE_PRIVATE, 2), (PLACE_PRIVATE, 4), (PLACE_PRIVATE, 8),
            (PLACE_WORKSPACE, 1), (PLACE_WORKSPACE, 2), (PLACE_WORKSPACE, 4),
-           (PLACE_LDS160, 1), (PLACE_LDS312, 1), (PLACE_LDS160_REG128, 1),
+           (PLACE_LDS160, 1), (PLACE_LDS312, 1), (PLACE_LDS160_REG112, 1),
            (PLACE_SPLIT2, 1), (PLACE_SPLIT4, 1)]
 SCHEDULES = ((SCHED_PERSISTENT, 64), (SCHED_PERSISTENT, 256), (SCHED_GRID, 64),
              (SCHED_GRID, 256), (SCHED_GRID, 512), (SCHED_GRID, 1024))
 REFILLS = (1, 2, 4, 8, 16)
-DEFAULT = dict(schedule=SCHED_PERSISTENT, placement=PLACE_LDS160_REG128, waves=1, block=64, refill=2)
+DEFAULT = dict(schedule=SCHED_PERSISTENT, placement=PLACE_LDS160_REG112, waves=1, block=64, refill=2)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "radm2_amd", "autotune.json")
 
 

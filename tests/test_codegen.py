@@ -82,14 +82,17 @@ def _formed(wide=False):
             G.recomputed_u(MECH, maxterms=t))
 
 
-PAIRS = [("rc", False, ""), ("rcw", True, "_WIDE")]
+# (name, wide, constant suffix, load batch) of the two decomposition/solve
+# pairs with formed entries
+PAIRS = [("rc", False, "", 16), ("rcw", True, "_WIDE", G.RC_BATCH_WIDE)]
 
 
-@pytest.mark.parametrize("fn", ["jac_decomp_pf16", "jac_decomp_rc_pf16", "jac_decomp_rcw_pf16"])
+@pytest.mark.parametrize("fn", ["jac_decomp_pf16", "jac_decomp_rc_pf16",
+                                "jac_decomp_rcw_pf%d" % G.RC_BATCH_WIDE])
 def test_lu_entries_are_read_after_they_are_stored(header, fn):
     body = _body(header, "int %s(" % fn)
     skipped = frozenset() if fn == "jac_decomp_pf16" else frozenset().union(
-        *_formed(wide=(fn == "jac_decomp_rcw_pf16")))
+        *_formed(wide=fn.startswith("jac_decomp_rcw_")))
     stored = {}
     assert len(re.findall(r"RDP<\d+>\(g\)", body)) > 300   # the check is not vacuous
     for n, ln in enumerate(body.split("\n")):
@@ -104,8 +107,9 @@ def test_lu_entries_are_read_after_they_are_stored(header, fn):
     assert sorted(stored) == sorted(set(range(MECH["lu"]["nnz"])) - skipped)
 
 
-@pytest.mark.parametrize("pair,wide,suffix", PAIRS)
-def test_recomputed_l_entries_are_formed_as_the_decomposition_forms_them(header, pair, wide, suffix):
+@pytest.mark.parametrize("pair,wide,suffix,batch", PAIRS)
+def test_recomputed_l_entries_are_formed_as_the_decomposition_forms_them(header, pair, wide, suffix,
+                                                                         batch):
     # the solves form these L entries instead of reading them back: each
     # must be an L position that no earlier elimination of its row updates,
     # never stored by the decomposition the kernels run, never read by their
@@ -117,7 +121,7 @@ def test_recomputed_l_entries_are_formed_as_the_decomposition_forms_them(header,
     assert len(rc) == int(re.search(r"RECOMPUTED_L%s = (\d+);" % suffix, header).group(1)) > 50
     row_of = {kk: k for k in range(59) for kk in range(crow[k], crow[k + 1])}
     dec = _body(header, "int jac_decomp_pf16(")
-    solve = _body(header, "void solve_%s_pf16_reg(" % pair)
+    solve = _body(header, "void solve_%s_pf%d_reg(" % (pair, batch))
     read = {int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", solve)}
     assert not (read & rc)
     # every other LU entry is read (the formed U entries are formed too)
@@ -183,8 +187,9 @@ def test_photolysis_map_matches_the_rate_laws():
     assert sorted(pj) == list(range(21))
 
 
-@pytest.mark.parametrize("pair,wide,suffix", PAIRS)
-def test_recomputed_u_entries_are_formed_as_the_decomposition_forms_them(header, pair, wide, suffix):
+@pytest.mark.parametrize("pair,wide,suffix,batch", PAIRS)
+def test_recomputed_u_entries_are_formed_as_the_decomposition_forms_them(header, pair, wide, suffix,
+                                                                         batch):
     # the decomposition and the solves also form U entries: each an off-diagonal U
     # position that no elimination of its row updates, so that it is the
     # Jacobian expression itself; the decomposition neither stores nor reads
@@ -196,8 +201,8 @@ def test_recomputed_u_entries_are_formed_as_the_decomposition_forms_them(header,
     assert len(rc_u) == int(re.search(r"RECOMPUTED_U%s = (\d+);" % suffix, header).group(1)) > 20
     row_of = {kk: k for k in range(59) for kk in range(crow[k], crow[k + 1])}
     dec0 = _body(header, "int jac_decomp_pf16(")
-    dec = _body(header, "int jac_decomp_%s_pf16(" % pair)
-    solve = _body(header, "void solve_%s_pf16_reg(" % pair)
+    dec = _body(header, "int jac_decomp_%s_pf%d(" % (pair, batch))
+    solve = _body(header, "void solve_%s_pf%d_reg(" % (pair, batch))
     read = {int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", solve)}
     assert read == set(range(lu["nnz"])) - rc_l - rc_u
     assert not ({int(q) for q in re.findall(r"RDP<(\d+)>\(g\)", dec)} & rc_u)

@@ -121,7 +121,7 @@ def test_lds_placements_reject_wide_blocks(solver, cells):
     sub = {k: np.ascontiguousarray(v[..., :64]) for k, v in cells.items()
            if k in ("conc", "temp", "cair", "h2o", "jrate", "rc_n2o5")}
     b = _batch(sub)
-    for pl in (radm2_amd.PLACE_LDS160, radm2_amd.PLACE_LDS312, radm2_amd.PLACE_LDS160_REG128):
+    for pl in (radm2_amd.PLACE_LDS160, radm2_amd.PLACE_LDS312, radm2_amd.PLACE_LDS160_REG112):
         solver.set_kernel(pl, 1)
         solver.set_schedule(radm2_amd.SCHED_PERSISTENT, 256, 0)
         with pytest.raises(radm2_amd.Radm2Error) as e:
