@@ -1078,8 +1078,11 @@ RC_MAXTERMS_WIDE = 3
 # 24 with 96 registers 522, 32 with 64 registers 525 (all bit-identical).
 # The family kernel: the tendency at 20 against 16, Ros4, Rodas3 and Rodas4
 # 2.3% faster, Ros2 1.2% and forced Ros3 0.4% slower; 20 in the general
-# stage loop only was slower on every method.
+# stage loop only was slower on every method.  The _rc pair (the one-shot
+# grid and the family kernel) at 20 against 16: the grid 758.5 -> 751.5 ms,
+# Ros2, Ros4, Rodas3, Rodas4 and forced Ros3 2.0-4.7% faster.
 RC_BATCH_WIDE = 20
+RC_BATCH = 20
 FUN_REG_BATCH = 20
 
 
@@ -1159,9 +1162,9 @@ def generate(dst=None, mech_path=None):
         "// L and U entries formed at their use instead of stored and read back",
         "constexpr int RECOMPUTED_L = %d;" % len(rc_l),
         "constexpr int RECOMPUTED_U = %d;" % len(rc_u),
-        gen_jac_decomp_pf(mech, 16, rc_l | rc_u, "jac_decomp_rc_pf16", rc_u)[0],
+        gen_jac_decomp_pf(mech, RC_BATCH, rc_l | rc_u, "jac_decomp_rc_pf%d" % RC_BATCH, rc_u)[0],
         "",
-        gen_solve_rc_pf_reg(mech, 16, rc_l, "solve_rc_pf16_reg", rc_u),
+        gen_solve_rc_pf_reg(mech, RC_BATCH, rc_l, "solve_rc_pf%d_reg" % RC_BATCH, rc_u),
         "",
         "constexpr int RECOMPUTED_L_WIDE = %d;" % len(rcw_l),
         "constexpr int RECOMPUTED_U_WIDE = %d;" % len(rcw_u),

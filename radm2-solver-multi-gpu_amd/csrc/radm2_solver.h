@@ -199,20 +199,20 @@ __device__ __forceinline__ int stage_jac_decomp(KA k, YA y, double h2o, double m
 // constants, concentrations and inverse pivots, by the same operations:
 // the state and the rate constants they use (no photolysis rate, which the
 // forcing moves between stages) are fixed through an attempt.  WIDE: the
-// larger set (entries of up to three Jacobian terms instead of two), its
-// loads issued 20 at a time instead of 16.
+// larger set (entries of up to three Jacobian terms instead of two).  Both
+// pairs issue their loads 20 at a time.
 template <bool WIDE, class KA, class YA, class GA>
 __device__ __forceinline__ int stage_decomp(KA k, YA y, double h2o, double m, double ghinv,
                                             GA g) {
   if constexpr (WIDE) return gen::jac_decomp_rcw_pf20(k, y, h2o, m, ghinv, g);
-  else return gen::jac_decomp_rc_pf16(k, y, h2o, m, ghinv, g);
+  else return gen::jac_decomp_rc_pf20(k, y, h2o, m, ghinv, g);
 }
 
 template <bool WIDE, class KA, class YA, class GA>
 __device__ __forceinline__ void stage_solve_reg(KA k, YA y, double h2o, double m, GA g,
                                                 double (&x)[NVAR]) {
   if constexpr (WIDE) gen::solve_rcw_pf20_reg(k, y, h2o, m, g, x);
-  else gen::solve_rc_pf16_reg(k, y, h2o, m, g, x);
+  else gen::solve_rc_pf20_reg(k, y, h2o, m, g, x);
 }
 
 template <int RF, class KA, class VA, class FA>

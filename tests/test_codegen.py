@@ -84,10 +84,10 @@ def _formed(wide=False):
 
 # (name, wide, constant suffix, load batch) of the two decomposition/solve
 # pairs with formed entries
-PAIRS = [("rc", False, "", 16), ("rcw", True, "_WIDE", G.RC_BATCH_WIDE)]
+PAIRS = [("rc", False, "", G.RC_BATCH), ("rcw", True, "_WIDE", G.RC_BATCH_WIDE)]
 
 
-@pytest.mark.parametrize("fn", ["jac_decomp_pf16", "jac_decomp_rc_pf16",
+@pytest.mark.parametrize("fn", ["jac_decomp_pf16", "jac_decomp_rc_pf%d" % G.RC_BATCH,
                                 "jac_decomp_rcw_pf%d" % G.RC_BATCH_WIDE])
 def test_lu_entries_are_read_after_they_are_stored(header, fn):
     body = _body(header, "int %s(" % fn)
