@@ -82,9 +82,10 @@ constexpr int kDefaultPersistentBlock = 64;
 // persistent schedule, explicit workspace placement with its load-scheduled
 // kernels, the 160 most accessed doubles per lane in LDS (two 64-lane
 // blocks per CU) and the next 112 in registers (1 wave per SIMD register
-// budget; 128 left no room for load batches of 20), refill once two lanes
-// of a wave are idle (1: 537.5 ms, 2: 531.7, 3: 534.1 at 1M cells).
-constexpr int kDefaultRefillMin = 2;
+// budget; 128 left no room for load batches of 20), refill once three
+// lanes of a wave are idle (1: 527.3 ms, 2: 521.2, 3: 517.6, 4: 519.5, 5:
+// 524.6, 6: 529.9 at 1M cells).
+constexpr int kDefaultRefillMin = 3;
 constexpr int kDefaultWaves = 1;
 constexpr int kDefaultPlacement = radm2::PLACE_HOT160R112;
 

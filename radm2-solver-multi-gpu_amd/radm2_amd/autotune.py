@@ -39,8 +39,8 @@ KERNELS = [(PLACE_PRIVATE, 1), (PLACE_PRIVATE, 2), (PLACE_PRIVATE, 4), (PLACE_PR
            (PLACE_SPLIT2, 1), (PLACE_SPLIT4, 1)]
 SCHEDULES = ((SCHED_PERSISTENT, 64), (SCHED_PERSISTENT, 256), (SCHED_GRID, 64),
              (SCHED_GRID, 256), (SCHED_GRID, 512), (SCHED_GRID, 1024))
-REFILLS = (1, 2, 4, 8, 16)
-DEFAULT = dict(schedule=SCHED_PERSISTENT, placement=PLACE_LDS160_REG112, waves=1, block=64, refill=2)
+REFILLS = (1, 2, 3, 4, 8, 16)
+DEFAULT = dict(schedule=SCHED_PERSISTENT, placement=PLACE_LDS160_REG112, waves=1, block=64, refill=3)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "radm2_amd", "autotune.json")
 
 
