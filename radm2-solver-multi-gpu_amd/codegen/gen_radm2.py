@@ -1059,13 +1059,14 @@ def gen_wrf_tables(mech):
 # LU entries the decomposition and the solves form instead of storing and
 # reading back (recomputed_l(), recomputed_u()): none that earlier
 # eliminations of its row update, at most RC_MAXTERMS Jacobian terms (the
-# _rc pair) or RC_MAXTERMS_WIDE (the _rcw pair, which the persistent Ros3
-# kernel runs).  Measured at 1M cells, default kernel: L only, updates
+# _rc pair, which the family kernel runs) or RC_MAXTERMS_WIDE (the _rcw
+# pair, which the Ros3 kernels run).  Measured at 1M cells, default kernel: L only, updates
 # allowed 0 / 1 / 2 / 8: 627.7 / 647 / 655 / 660 ms (all terms); terms at
 # most 1 / 2 / any: 623.5 / 621.8 / 627.7 (no updates); none formed: 647.
 # With the U entries and the current build: terms at most 2 / 3 / 5:
-# 540.7 / 537.1 / 544.4 ms; one update allowed: 559.  The one-shot grid and
-# the family kernel are faster with 2 (775 against 798 ms; 2-5%).
+# 540.7 / 537.1 / 544.4 ms; one update allowed: 559.  The family kernel is
+# faster with 2 (2-5%); so was the one-shot grid (775 against 798 ms) until
+# the load batches of 20, with which it gains from 3 (751.9 -> 738.9 ms).
 RC_MAXMOD = 0
 RC_MAXTERMS = 2
 RC_MAXTERMS_WIDE = 3

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(BLOCK, RADM2_WAVES) void k_ros3_grid(LaunchArgs a) 
     if (run) {
       cell_attempt_top(s, o);
       if (s.status == ST_RUNNING)
-        ros3_attempt<RADM2_LROWS, RADM2_METHODS != 0, false>(s, o, fc, kk, y, h2o, m, lu, f, k1, k2, k3);
+        ros3_attempt<RADM2_LROWS, RADM2_METHODS != 0, RADM2_METHODS == 0>(s, o, fc, kk, y, h2o, m, lu, f, k1, k2, k3);
     }
   }
   if (valid) cell_store(d, p, y, s, s.hfirst, a.stats);

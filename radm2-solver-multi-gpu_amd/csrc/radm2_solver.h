@@ -518,10 +518,11 @@ __device__ __forceinline__ double generic_stages(const RosMethod& rm, const Forc
 // R: launder interval of the compiler-scheduled fused Jacobian/decomposition
 // (rows) and, as 16 reactions per unit, of the tendency on private arrays;
 // phases are laundered apart whenever the arrays are workspace views.
-// WIDE: the persistent Ros3 kernel's tuning of the fused stages (the wider
-// set of formed LU entries, and F2 + C31/H K1 kept for the third stage
-// instead of F2, which saves reloading K1 there); the one-shot grid and the
-// family kernel were faster without either (measured at 1M cells).
+// WIDE: the Ros3 kernels' tuning of the fused stages (the wider set of
+// formed LU entries, and F2 + C31/H K1 kept for the third stage instead of
+// F2, which saves reloading K1 there); the family kernel, whose general
+// stage loop shares the narrower set, was faster without either (measured
+// at 1M cells; the one-shot grid was too until the load batches of 20).
 // FAMILY: the kernel runs every method of KPP's Rosenbrock family, the
 // non-autonomous forcing fc and KPP's step halving after a singular matrix
 // (workspace views with seven stage regions); otherwise Ros3 alone,
