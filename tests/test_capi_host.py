@@ -37,6 +37,23 @@ def test_library_exports_every_declared_symbol():
     assert sorted(radm2_amd.EXPORTED_SYMBOLS + mlp.EXPORTED_SYMBOLS) == names
 
 
+def test_schedule_and_placement_constants_match_the_python_mirror():
+    # every RADM2_SCHED_* and RADM2_PLACE_* value of the header (an alias
+    # resolved to its target) equals the Python constant of the same name
+    import radm2_amd
+    text = open(os.path.join(ROOT, "include", "radm2.h")).read()
+    defs = dict(re.findall(r"^#define (RADM2_(?:SCHED|PLACE)_\w+) (\w+)", text, re.M))
+    vals = {}
+    for name, v in defs.items():
+        while v in defs:
+            v = defs[v]
+        vals[name] = int(v)
+    assert len(vals) == 10
+    assert vals["RADM2_PLACE_LDS160_REG128"] == vals["RADM2_PLACE_LDS160_REG112"] == 4
+    for name, v in vals.items():
+        assert getattr(radm2_amd, name[len("RADM2_"):]) == v, name
+
+
 def test_strerror_texts_follow_kpp():
     import radm2_amd
     assert radm2_amd.strerror(-5) == "Improper tolerance values"
